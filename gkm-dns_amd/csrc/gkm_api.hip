@@ -1,0 +1,367 @@
+// gkm_api.hip — the C ABI of libgkm.so (declared in include/gkm.h).
+// Host code only: argument checks, device allocation, copies and launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gkm_internal.h"
+
+using gkm::Ctx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(GKM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int check_params(const gkm_grid* g, const gkm_params* p) {
+    if (!g || !p) return fail(GKM_E_ARG, "null grid or params");
+    if (g->ghost != 3) return fail(GKM_E_ARG, "ghost must be 3 (got %d)", g->ghost);
+    if (g->nx < 3 || g->ny < 3 || g->nz < 3)
+        return fail(GKM_E_ARG, "box %dx%dx%d too small: each axis needs at least 3 interior cells", g->nx, g->ny,
+                    g->nz);
+    const double padded = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 6);
+    if (padded > 2.0e9) return fail(GKM_E_ARG, "box too large: %.0f padded cells", padded);
+    if (!(g->dx > 0) || !std::isfinite(g->dx)) return fail(GKM_E_ARG, "dx must be positive");
+    if (!(p->gamma > 1.0)) return fail(GKM_E_ARG, "gamma must exceed 1");
+    if (!(p->R > 0) || !(p->Pr > 0) || !(p->mu0 >= 0) || !(p->T0 > 0))
+        return fail(GKM_E_ARG, "R, Pr, T0 must be positive and mu0 non-negative");
+    if (p->K < 0) return fail(GKM_E_ARG, "K must be non-negative");
+    if (p->visc_law != GKM_VISC_SUTHERLAND && p->visc_law != GKM_VISC_POWER) return fail(GKM_E_ARG, "bad visc_law");
+    if (p->recon != GKM_RECON_SYMOO && p->recon != GKM_RECON_JS) return fail(GKM_E_ARG, "bad recon");
+    if (p->energy_recon != GKM_ENERGY_T && p->energy_recon != GKM_ENERGY_E) return fail(GKM_E_ARG, "bad energy_recon");
+    if (p->precision != GKM_PREC_FP64 && p->precision != GKM_PREC_MIXED) return fail(GKM_E_ARG, "bad precision");
+    return GKM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gkm_last_error(void) { return g_err.c_str(); }
+
+const char* gkm_version(void) { return "gkm-mi355x 0.1 gfx950"; }
+
+int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
+    if (!out) return fail(GKM_E_ARG, "null output handle");
+    *out = nullptr;
+    int rc = check_params(grid, params);
+    if (rc) return rc;
+    Ctx* h = new (std::nothrow) Ctx();
+    if (!h) return fail(GKM_E_NOMEM, "host allocation failed");
+    h->grid = *grid;
+    h->par = *params;
+    gkm::Geo& g = h->geo;
+    g.nx = grid->nx;
+    g.ny = grid->ny;
+    g.nz = grid->nz;
+    g.P0 = grid->ny + 6;
+    g.P1 = grid->nx + 6;
+    g.P2 = grid->nz + 6;
+    g.s1 = g.P2;
+    g.s0 = (long)g.P1 * g.P2;
+    h->ncell = (long)g.P0 * g.P1 * g.P2;
+    h->flux_block = 256;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(GKM_E_HIP, "no usable HIP device: %s", hipGetErrorString(e));
+    }
+    const size_t n = (size_t)h->ncell;
+    const size_t align = 256;
+    auto up = [&](size_t b) { return (b + align - 1) / align * align; };
+    const bool fp64 = params->precision == GKM_PREC_FP64;
+    const size_t plane = up(n * (fp64 ? sizeof(double) : sizeof(float)));
+    const int nplanes = fp64 ? (5 + 10 + 15) : (10 + 10 + 15);
+    h->red_blocks = 2048;
+    const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
+    h->arena_bytes = plane * nplanes + red;
+    e = hipMalloc(&h->arena, h->arena_bytes);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return fail(GKM_E_NOMEM, "hipMalloc of %zu bytes failed: %s", (size_t)0 + n * nplanes, hipGetErrorString(e));
+    }
+    char* base = (char*)h->arena;
+    auto take = [&](size_t b) {
+        char* p = base;
+        base += b;
+        return p;
+    };
+    if (fp64) {
+        for (int q = 0; q < 5; q++) h->Wd[q] = (double*)take(plane);
+        for (int q = 0; q < 10; q++) h->WId[q] = (double*)take(plane);
+        for (int q = 0; q < 15; q++) h->Fd[q] = (double*)take(plane);
+    } else {
+        for (int q = 0; q < 5; q++) h->Whi[q] = (float*)take(plane);
+        for (int q = 0; q < 5; q++) h->Wlo[q] = (float*)take(plane);
+        for (int q = 0; q < 10; q++) h->WIf[q] = (float*)take(plane);
+        for (int q = 0; q < 15; q++) h->Ff[q] = (float*)take(plane);
+    }
+    h->red_part = (double*)take(up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA));
+    h->red_out = (double*)take(up(sizeof(double) * 64));
+    // interface and flux planes start at zero so that unused border entries are defined
+    e = hipMemsetAsync(h->arena, 0, h->arena_bytes, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(h->arena);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return fail(GKM_E_HIP, "initial clear failed: %s", hipGetErrorString(e));
+    }
+    *out = (gkm_t*)h;
+    return GKM_OK;
+}
+
+int gkm_destroy(gkm_t* hh) {
+    Ctx* h = (Ctx*)hh;
+    if (!h) return GKM_OK;
+    (void)hipStreamSynchronize(h->stream);
+    if (h->events_made)
+        for (auto& ev : h->ev_step) (void)hipEventDestroy(ev);
+    (void)hipFree(h->arena);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return GKM_OK;
+}
+
+int gkm_set_state(gkm_t* hh, const double* const W[5]) {
+    Ctx* h = (Ctx*)hh;
+    if (!h || !W) return fail(GKM_E_ARG, "null handle or state");
+    for (int q = 0; q < 5; q++)
+        if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
+    const size_t bytes = sizeof(double) * (size_t)h->ncell;
+    if (h->par.precision == GKM_PREC_FP64) {
+        for (int q = 0; q < 5; q++) HIP_TRY(hipMemcpyAsync(h->Wd[q], W[q], bytes, hipMemcpyHostToDevice, h->stream));
+    } else {
+        // stage through the (float) flux planes: 15 float planes hold >= 1 double plane
+        double* stage = (double*)h->Ff[0];
+        for (int q = 0; q < 5; q++) {
+            HIP_TRY(hipMemcpyAsync(stage, W[q], bytes, hipMemcpyHostToDevice, h->stream));
+            gkm::launch_split(h, q, stage);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return GKM_OK;
+}
+
+int gkm_get_state(gkm_t* hh, double* const W[5]) {
+    Ctx* h = (Ctx*)hh;
+    if (!h || !W) return fail(GKM_E_ARG, "null handle or state");
+    for (int q = 0; q < 5; q++)
+        if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
+    const size_t bytes = sizeof(double) * (size_t)h->ncell;
+    if (h->par.precision == GKM_PREC_FP64) {
+        for (int q = 0; q < 5; q++) HIP_TRY(hipMemcpyAsync(W[q], h->Wd[q], bytes, hipMemcpyDeviceToHost, h->stream));
+    } else {
+        double* stage = (double*)h->Ff[0];
+        for (int q = 0; q < 5; q++) {
+            gkm::launch_merge(h, q, stage);
+            HIP_TRY(hipMemcpyAsync(W[q], stage, bytes, hipMemcpyDeviceToHost, h->stream));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return GKM_OK;
+}
+
+int gkm_fill_ghosts(gkm_t* hh) {
+    Ctx* h = (Ctx*)hh;
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    gkm::launch_ghost(h);
+    HIP_TRY(hipGetLastError());
+    return GKM_OK;
+}
+
+int gkm_step(gkm_t* hh, double dt, int nsteps) {
+    Ctx* h = (Ctx*)hh;
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(GKM_E_ARG, "dt must be positive and finite");
+    if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
+    if (h->timing && !h->events_made) {
+        for (auto& ev : h->ev_step) HIP_TRY(hipEventCreate(&ev));
+        h->events_made = true;
+    }
+    h->timed_steps = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const bool timed = h->timing && s < gkm::kTimedSteps;
+        gkm::launch_step(h, dt, timed, s);
+        if (timed) h->timed_steps = s + 1;
+    }
+    gkm::launch_ghost(h);
+    HIP_TRY(hipGetLastError());
+    return GKM_OK;
+}
+
+int gkm_sync(gkm_t* hh) {
+    Ctx* h = (Ctx*)hh;
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return GKM_OK;
+}
+
+int gkm_set_timing(gkm_t* hh, int enabled) {
+    Ctx* h = (Ctx*)hh;
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    h->timing = enabled != 0;
+    return GKM_OK;
+}
+
+int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
+    Ctx* h = (Ctx*)hh;
+    if (!h || !ms) return fail(GKM_E_ARG, "null argument");
+    for (int i = 0; i < 4; i++) ms[i] = -1.0;
+    if (nsteps) *nsteps = h->timed_steps;
+    if (!h->timing || h->timed_steps == 0) return GKM_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double flux = 0, recon = 0, ghost = 0, upd = 0;
+    for (int s = 0; s < h->timed_steps; s++) {
+        hipEvent_t* ev = h->ev_step + s * 12;
+        float t;
+        HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+        ghost += t;
+        for (int d = 0; d < 3; d++) {
+            HIP_TRY(hipEventElapsedTime(&t, ev[1 + 3 * d], ev[2 + 3 * d]));
+            recon += t;
+            HIP_TRY(hipEventElapsedTime(&t, ev[2 + 3 * d], ev[3 + 3 * d]));
+            flux += t;
+        }
+        HIP_TRY(hipEventElapsedTime(&t, ev[9], ev[11]));
+        upd += t;
+    }
+    ms[0] = flux;
+    ms[1] = recon;
+    ms[2] = ghost;
+    ms[3] = upd;
+    return GKM_OK;
+}
+
+int gkm_compute_dt(gkm_t* hh, double cfl, double* dt) {
+    Ctx* h = (Ctx*)hh;
+    if (!h || !dt) return fail(GKM_E_ARG, "null argument");
+    if (!(cfl > 0)) return fail(GKM_E_ARG, "cfl must be positive");
+    double a[gkm::kStatA];
+    HIP_TRY(gkm::launch_stats_a(h, a));
+    const double mx = a[27];
+    if (!(mx > 0) || !std::isfinite(mx)) return fail(GKM_E_NONPHYSICAL, "signal speed %g is not positive/finite", mx);
+    *dt = cfl * h->grid.dx / mx;
+    return GKM_OK;
+}
+
+int gkm_stats(gkm_t* hh, gkm_stats_t* out) {
+    Ctx* h = (Ctx*)hh;
+    if (!h || !out) return fail(GKM_E_ARG, "null argument");
+    gkm::launch_ghost(h);
+    double a[gkm::kStatA];
+    HIP_TRY(gkm::launch_stats_a(h, a));
+    const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
+    gkm_stats_t s;
+    std::memset(&s, 0, sizeof s);
+    s.rho_mean = a[0] / N;
+    s.p_mean = a[1] / N;
+    s.T_mean = a[2] / N;
+    s.v_mean = a[3] / N;
+    s.c_mean = a[4] / N;
+    s.q2 = a[5] / N;
+    s.tke = a[6] / N;
+    s.ek = 0.5 * s.tke;
+    s.mt = std::sqrt(s.q2) / s.c_mean;
+    s.mass = a[7];
+    s.mom_x = a[8];
+    s.mom_y = a[9];
+    s.mom_z = a[10];
+    s.energy = a[11];
+    s.dudx2 = a[12] / N; s.dudx3 = a[13] / N; s.dudx4 = a[14] / N;
+    s.dvdy2 = a[15] / N; s.dvdy3 = a[16] / N; s.dvdy4 = a[17] / N;
+    s.dwdz2 = a[18] / N; s.dwdz3 = a[19] / N; s.dwdz4 = a[20] / N;
+    s.enstrophy = a[21] / N;
+    s.dil2 = a[22] / N;
+    s.diss_sol = a[23] / N;
+    s.diss_dil = (4.0 / 3.0) * a[24] / N;
+    s.diss_ref = a[25] / N;
+    s.sijsij = a[26] / N;
+    s.umax_plus_c = a[27];
+    s.skew_u = s.dudx3 / std::pow(s.dudx2, 1.5);
+    s.skew_v = s.dvdy3 / std::pow(s.dvdy2, 1.5);
+    s.skew_w = s.dwdz3 / std::pow(s.dwdz2, 1.5);
+    s.kurt_u = s.dudx4 / (s.dudx2 * s.dudx2);
+    s.kurt_v = s.dvdy4 / (s.dvdy2 * s.dvdy2);
+    s.kurt_w = s.dwdz4 / (s.dwdz2 * s.dwdz2);
+    const double m[4] = {s.rho_mean, s.p_mean, s.T_mean, s.v_mean};
+    double b[gkm::kStatB];
+    HIP_TRY(gkm::launch_stats_b(h, m, b));
+    s.rho_rms = std::sqrt(b[0] / N);
+    s.p_rms = std::sqrt(b[1] / N);
+    s.T_rms = std::sqrt(b[2] / N);
+    s.v_rms = std::sqrt(b[3] / N);
+    *out = s;
+    if (!(s.rho_mean > 0) || !(s.p_mean > 0) || !std::isfinite(s.energy))
+        return fail(GKM_E_NONPHYSICAL, "non-physical state: <rho>=%g <p>=%g", s.rho_mean, s.p_mean);
+    return GKM_OK;
+}
+
+static int points_common(const gkm_params* p, int n, const double* in, size_t in_per, double* out, size_t out_per,
+                         double** din, double** dout, hipStream_t* s) {
+    if (!p || !in || !out || n < 0) return fail(GKM_E_ARG, "null argument or negative count");
+    HIP_TRY(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(din, sizeof(double) * in_per * (size_t)(n > 0 ? n : 1)));
+    HIP_TRY(hipMalloc(dout, sizeof(double) * out_per * (size_t)(n > 0 ? n : 1)));
+    HIP_TRY(hipMemcpyAsync(*din, in, sizeof(double) * in_per * n, hipMemcpyHostToDevice, *s));
+    return GKM_OK;
+}
+
+int gkm_flux_points(const gkm_params* p, int n, const double* in, double dx, double* F) {
+    double *din = nullptr, *dout = nullptr;
+    hipStream_t s = nullptr;
+    int rc = points_common(p, n, in, 52, F, 5, &din, &dout, &s);
+    if (rc == GKM_OK && n > 0) {
+        gkm::launch_flux_points(s, p->precision, n, din, dx, p->gamma, 1.0 / p->Pr, p->K, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(F, dout, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(GKM_E_HIP, "flux points: %s", hipGetErrorString(e));
+    }
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int gkm_weno_points(const gkm_params* p, int n, const double* in, double* out) {
+    double *din = nullptr, *dout = nullptr;
+    hipStream_t s = nullptr;
+    int rc = points_common(p, n, in, 30, out, 10, &din, &dout, &s);
+    if (rc == GKM_OK && n > 0) {
+        gkm::launch_weno_points(s, p->precision, p->recon, n, din, p->gamma, p->R, p->energy_recon, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(GKM_E_HIP, "weno points: %s", hipGetErrorString(e));
+    }
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
+}  // extern "C"

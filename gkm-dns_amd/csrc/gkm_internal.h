@@ -1,0 +1,60 @@
+// gkm_internal.h — handle layout and launcher declarations shared by the
+// translation units of libgkm.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/gkm.h"
+#include "gkm_kernels.h"
+
+namespace gkm {
+
+constexpr int kTimedSteps = 64;  // steps per gkm_step call that carry HIP events
+
+struct Ctx {
+    gkm_grid grid;
+    gkm_params par;
+    Geo geo;
+    long ncell;  // padded cells per plane
+    int device;
+    hipStream_t stream;
+    int flux_block;
+    // FP64 form
+    double* Wd[5] = {};
+    double* WId[10] = {};
+    double* Fd[15] = {};
+    // mixed form
+    float* Whi[5] = {};
+    float* Wlo[5] = {};
+    float* WIf[10] = {};
+    float* Ff[15] = {};
+    // one device allocation holds every plane
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    // reductions
+    double* red_part = nullptr;  // partial sums per block
+    double* red_out = nullptr;   // final values
+    int red_blocks = 0;
+    // timing
+    bool timing = false;
+    int timed_steps = 0;
+    hipEvent_t ev_step[kTimedSteps * 12];
+    bool events_made = false;
+};
+
+StepPar step_par(const Ctx* h, double dt);
+void launch_ghost(Ctx* h);
+void launch_step(Ctx* h, double dt, bool timed, int step_index);
+void launch_split(Ctx* h, int q, const double* dsrc);
+void launch_merge(Ctx* h, int q, double* ddst);
+void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,
+                        double* out);
+void launch_weno_points(hipStream_t s, int prec, int recon, int n, const double* rec, double gam, double R,
+                        int energy_recon, double* out);
+
+// statistics: number of accumulated quantities of each pass
+constexpr int kStatA = 28;  // sums + one max (slot 27)
+constexpr int kStatB = 4;
+hipError_t launch_stats_a(Ctx* h, double* host_out /*kStatA*/);
+hipError_t launch_stats_b(Ctx* h, const double means[4], double* host_out /*kStatB*/);
+
+}  // namespace gkm

@@ -1,0 +1,238 @@
+// gkm_stats.hip — FP64 reductions over the interior: CFL signal speed and the
+// turbulence statistics (replaces timestep, weno/src/update.cpp:1057-1079, the
+// sums of PrintStats, update.cpp:638-784, and the GKM on-device TKE / rms
+// reductions, GKM/CUDA GKM/src/GPUupdate.cu:425-483,1742-1783).
+//
+// Each pass: a grid-stride kernel accumulates per-thread FP64 values, reduces
+// them across the 64-wide wavefront with shuffles and across the block in LDS,
+// and writes one partial per block; a one-block kernel then reduces the
+// partials in a fixed order, so the result does not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include "gkm_internal.h"
+
+namespace gkm {
+
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+
+// Reduce NQ per-thread values over the block; thread 0 writes the partials.
+// Slots listed in MAXMASK are max-reduced, the others summed.
+template <int NQ, unsigned MAXMASK>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NQ], double* part) {
+    __shared__ double lds[NQ][kRedThreads / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        double v = ((MAXMASK >> q) & 1u) ? wave_max(acc[q]) : wave_sum(acc[q]);
+        if (lane == 0) lds[q][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        double v = lds[q][0];
+        for (int w = 1; w < kRedThreads / 64; w++) v = ((MAXMASK >> q) & 1u) ? fmax(v, lds[q][w]) : v + lds[q][w];
+        part[(long)blockIdx.x * NQ + q] = v;
+    }
+}
+
+template <class St>
+__device__ __forceinline__ void vel(const St& st, long c, double u[3]) {
+    const double r = st.get(0, c);
+    u[0] = st.get(1, c) / r;
+    u[1] = st.get(2, c) / r;
+    u[2] = st.get(3, c) / r;
+}
+
+struct StatPar {
+    double gam, R, mu0, T0, visc_exp, dx;
+    int visc_law;
+};
+
+__device__ __forceinline__ double visc(const StatPar& p, double T) {
+    if (p.visc_law == 1) return p.mu0 * pow(T / p.T0, p.visc_exp);
+    return p.mu0 * pow(T / p.T0, 1.5) * (p.T0 + 110.4) / (T + 110.4);
+}
+
+// pass A: slot layout
+//  0 rho  1 p  2 T  3 1/rho  4 c  5 |u|^2  6 rho|u|^2  7..11 W0..W4
+// 12..14 (du/dx)^2,3,4  15..17 (dv/dy)^n  18..20 (dw/dz)^n
+// 21 |curl u|^2  22 (div u)^2  23 mu|curl u|^2  24 mu (div u)^2
+// 25 mu du_i/dx_j du_i/dx_j  26 S_ij S_ij (PrintStats form)  27 max(|u|+c)
+template <class St>
+__global__ void __launch_bounds__(kRedThreads) k_stats_a(Geo g, St st, StatPar p, double* part) {
+    double acc[kStatA];
+#pragma unroll
+    for (int q = 0; q < kStatA; q++) acc[q] = 0.0;
+    const long n = (long)g.nx * g.ny * g.nz;
+    const double i2dx = 1.0 / (2.0 * p.dx);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long r = t / g.nz;
+        const int i = (int)(r % g.nx);
+        const int j = (int)(r / g.nx);
+        const long c = (long)(j + 3) * g.s0 + (long)(i + 3) * g.s1 + (k + 3);
+        const double rho = st.get(0, c);
+        double u[3];
+        vel(st, c, u);
+        const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
+        const double T = P / (p.R * rho);
+        const double cs = sqrt(p.gam * P / rho);
+        acc[0] += rho; acc[1] += P; acc[2] += T; acc[3] += 1.0 / rho; acc[4] += cs;
+        acc[5] += q2; acc[6] += rho * q2;
+#pragma unroll
+        for (int q = 0; q < 5; q++) acc[7 + q] += st.get(q, c);
+        double d[3][3];
+        const long sa[3] = {g.s1, g.s0, 1};  // d/dx, d/dy, d/dz
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double up[3], um[3];
+            vel(st, c + sa[a], up);
+            vel(st, c - sa[a], um);
+#pragma unroll
+            for (int m = 0; m < 3; m++) d[m][a] = (up[m] - um[m]) * i2dx;
+        }
+        const double mu = visc(p, T);
+        const double dudx = d[0][0], dudy = d[0][1], dudz = d[0][2];
+        const double dvdx = d[1][0], dvdy = d[1][1], dvdz = d[1][2];
+        const double dwdx = d[2][0], dwdy = d[2][1], dwdz = d[2][2];
+        const double ox = dwdy - dvdz, oy = dudz - dwdx, oz = dvdx - dudy;
+        const double om2 = ox * ox + oy * oy + oz * oz;
+        const double th = dudx + dvdy + dwdz;
+        double gg = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) gg += d[a][b] * d[a][b];
+        double x2 = dudx * dudx; acc[12] += x2; acc[13] += x2 * dudx; acc[14] += x2 * x2;
+        x2 = dvdy * dvdy; acc[15] += x2; acc[16] += x2 * dvdy; acc[17] += x2 * x2;
+        x2 = dwdz * dwdz; acc[18] += x2; acc[19] += x2 * dwdz; acc[20] += x2 * x2;
+        acc[21] += om2; acc[22] += th * th; acc[23] += mu * om2; acc[24] += mu * th * th; acc[25] += mu * gg;
+        acc[26] += dudx * dudx + 0.5 * (dudy + dvdx) * (dudy + dvdx) + 0.5 * (dudz + dwdx) * (dudz + dwdx) +
+                   0.5 * (dwdy + dvdz) * (dwdy + dvdz) + dvdy * dvdy + dwdz * dwdz;
+        acc[27] = fmax(acc[27], sqrt(q2) + cs);
+    }
+    block_reduce_store<kStatA, (1u << 27)>(acc, part);
+}
+
+// pass B: squared deviations of rho, p, T, 1/rho about the given means
+template <class St>
+__global__ void __launch_bounds__(kRedThreads) k_stats_b(Geo g, St st, StatPar p, double m0, double m1, double m2,
+                                                         double m3, double* part) {
+    double acc[kStatB] = {0.0, 0.0, 0.0, 0.0};
+    const long n = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long r = t / g.nz;
+        const int i = (int)(r % g.nx);
+        const int j = (int)(r / g.nx);
+        const long c = (long)(j + 3) * g.s0 + (long)(i + 3) * g.s1 + (k + 3);
+        const double rho = st.get(0, c);
+        double u[3];
+        vel(st, c, u);
+        const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
+        const double T = P / (p.R * rho);
+        acc[0] += (rho - m0) * (rho - m0);
+        acc[1] += (P - m1) * (P - m1);
+        acc[2] += (T - m2) * (T - m2);
+        acc[3] += (1.0 / rho - m3) * (1.0 / rho - m3);
+    }
+    block_reduce_store<kStatB, 0u>(acc, part);
+}
+
+// final pass: one block reduces nb partial rows of NQ slots in block order
+template <int NQ, unsigned MAXMASK>
+__global__ void __launch_bounds__(kRedThreads) k_reduce_final(int nb, const double* part, double* out) {
+    for (int q = 0; q < NQ; q++) {
+        const bool ismax = (MAXMASK >> q) & 1u;
+        double v = ismax ? 0.0 : 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const double x = part[(long)b * NQ + q];
+            v = ismax ? fmax(v, x) : v + x;
+        }
+        v = ismax ? wave_max(v) : wave_sum(v);
+        __shared__ double lds[kRedThreads / 64];
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = lds[0];
+            for (int w = 1; w < kRedThreads / 64; w++) s = ismax ? fmax(s, lds[w]) : s + lds[w];
+            out[q] = s;
+        }
+        __syncthreads();
+    }
+}
+
+static StatPar stat_par(const Ctx* h) {
+    StatPar p;
+    p.gam = h->par.gamma;
+    p.R = h->par.R;
+    p.mu0 = h->par.mu0;
+    p.T0 = h->par.T0;
+    p.visc_exp = h->par.visc_exp;
+    p.dx = h->grid.dx;
+    p.visc_law = h->par.visc_law;
+    return p;
+}
+
+hipError_t launch_stats_a(Ctx* h, double* host_out) {
+    const Geo& g = h->geo;
+    const StatPar p = stat_par(h);
+    const int nb = h->red_blocks;
+    if (h->par.precision == GKM_PREC_FP64) {
+        StateD st;
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        hipLaunchKernelGGL(k_stats_a<StateD>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, h->red_part);
+    } else {
+        StateM st;
+        for (int q = 0; q < 5; q++) {
+            st.hi[q] = h->Whi[q];
+            st.lo[q] = h->Wlo[q];
+        }
+        hipLaunchKernelGGL(k_stats_a<StateM>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, h->red_part);
+    }
+    hipLaunchKernelGGL((k_reduce_final<kStatA, (1u << 27)>), dim3(1), dim3(kRedThreads), 0, h->stream, nb,
+                       h->red_part, h->red_out);
+    hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kStatA, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(h->stream);
+}
+
+hipError_t launch_stats_b(Ctx* h, const double m[4], double* host_out) {
+    const Geo& g = h->geo;
+    const StatPar p = stat_par(h);
+    const int nb = h->red_blocks;
+    if (h->par.precision == GKM_PREC_FP64) {
+        StateD st;
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        hipLaunchKernelGGL(k_stats_b<StateD>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
+                           m[3], h->red_part);
+    } else {
+        StateM st;
+        for (int q = 0; q < 5; q++) {
+            st.hi[q] = h->Whi[q];
+            st.lo[q] = h->Wlo[q];
+        }
+        hipLaunchKernelGGL(k_stats_b<StateM>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
+                           m[3], h->red_part);
+    }
+    hipLaunchKernelGGL((k_reduce_final<kStatB, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, nb, h->red_part,
+                       h->red_out);
+    hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kStatB, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(h->stream);
+}
+
+}  // namespace gkm

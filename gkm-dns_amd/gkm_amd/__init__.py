@@ -1,0 +1,9 @@
+"""gkm_amd — MI355X-native GKM-DNS time step (WENO + BGK flux + update,
+periodic ghost fill, CFL and statistics reductions) behind the C ABI of
+libgkm.so (include/gkm.h)."""
+from ._abi import LIB_PATH, GkmError, load  # noqa: F401
+from .ic import DhitIC, conservative_padded, dhit, dhit_velocity  # noqa: F401
+from .solver import GasParams, Solver, flux_points, weno_points  # noqa: F401
+
+__all__ = ["Solver", "GasParams", "dhit", "dhit_velocity", "conservative_padded", "DhitIC", "flux_points",
+           "weno_points", "GkmError", "load", "LIB_PATH"]

@@ -1,0 +1,116 @@
+"""ctypes binding of libgkm.so (the C ABI declared in include/gkm.h).
+
+The library is built in-tree (``make -C gkm-dns_amd/csrc``) and loaded from
+this folder.  There is no fallback: if the library is missing or no HIP device
+is usable, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libgkm.so"
+
+GKM_OK = 0
+GKM_E_ARG = -1
+GKM_E_HIP = -2
+GKM_E_NONPHYSICAL = -3
+GKM_E_NOMEM = -4
+GKM_E_UNSUPPORTED = -5
+
+VISC = {"sutherland": 0, "power": 1}
+RECON = {"symoo": 0, "js": 1}
+ENERGY = {"T": 0, "E": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+DP = C.POINTER(C.c_double)
+
+# every entry point include/gkm.h declares
+EXPORTS = (
+    "gkm_create", "gkm_destroy", "gkm_set_state", "gkm_get_state", "gkm_fill_ghosts", "gkm_step",
+    "gkm_compute_dt", "gkm_stats", "gkm_sync", "gkm_set_timing", "gkm_last_timing", "gkm_flux_points",
+    "gkm_weno_points", "gkm_last_error", "gkm_version",
+)
+
+
+class GkmGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ghost", C.c_int), ("dx", C.c_double)]
+
+
+class GkmParams(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu0", C.c_double),
+        ("T0", C.c_double), ("visc_exp", C.c_double), ("K", C.c_int), ("visc_law", C.c_int),
+        ("recon", C.c_int), ("energy_recon", C.c_int), ("zbug_compat", C.c_int), ("precision", C.c_int),
+    ]
+
+
+STAT_FIELDS = (
+    "tke", "ek", "q2", "mt", "rho_mean", "rho_rms", "p_mean", "p_rms", "T_mean", "T_rms",
+    "v_mean", "v_rms", "c_mean", "umax_plus_c",
+    "dudx2", "dudx3", "dudx4", "dvdy2", "dvdy3", "dvdy4", "dwdz2", "dwdz3", "dwdz4",
+    "skew_u", "skew_v", "skew_w", "kurt_u", "kurt_v", "kurt_w",
+    "enstrophy", "dil2", "diss_sol", "diss_dil", "diss_ref", "sijsij",
+    "mass", "mom_x", "mom_y", "mom_z", "energy",
+)
+
+
+class GkmStats(C.Structure):
+    _fields_ = [(f, C.c_double) for f in STAT_FIELDS]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f in STAT_FIELDS}
+
+
+class GkmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gkm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libgkm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"libgkm.so is not built ({LIB_PATH}); run `make -C gkm-dns_amd/csrc`")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.gkm_create.argtypes = [C.POINTER(GkmGrid), C.POINTER(GkmParams), C.POINTER(vp)]
+    L.gkm_destroy.argtypes = [vp]
+    L.gkm_set_state.argtypes = [vp, C.POINTER(DP)]
+    L.gkm_get_state.argtypes = [vp, C.POINTER(DP)]
+    L.gkm_fill_ghosts.argtypes = [vp]
+    L.gkm_step.argtypes = [vp, C.c_double, C.c_int]
+    L.gkm_compute_dt.argtypes = [vp, C.c_double, DP]
+    L.gkm_stats.argtypes = [vp, C.POINTER(GkmStats)]
+    L.gkm_sync.argtypes = [vp]
+    L.gkm_set_timing.argtypes = [vp, C.c_int]
+    L.gkm_last_timing.argtypes = [vp, DP, C.POINTER(C.c_int)]
+    L.gkm_flux_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
+    L.gkm_weno_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
+    L.gkm_last_error.restype = C.c_char_p
+    L.gkm_version.restype = C.c_char_p
+    for name in EXPORTS:
+        if name not in ("gkm_last_error", "gkm_version"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != GKM_OK:
+        raise GkmError(rc, load().gkm_last_error().decode(errors="replace"))
+
+
+def ptr_array(arrays) -> C.Array:
+    arr = (DP * len(arrays))()
+    for q, a in enumerate(arrays):
+        if a.dtype.name != "float64" or not a.flags.c_contiguous:
+            raise ValueError("state planes must be C-contiguous float64")
+        arr[q] = a.ctypes.data_as(DP)
+    return arr

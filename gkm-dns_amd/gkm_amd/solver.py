@@ -1,0 +1,236 @@
+"""Host-side mirror of the reference solver interface over libgkm.so.
+
+The reference drives one rank's segment through free functions and globals
+(weno/src/Main.cpp:10-49, FlowField.cpp, update.cpp):
+
+    GetParams()            -> Solver(...) / Solver.from_dhit(...)
+    Initialize_DHIT(W)     -> Solver.initialize_dhit(ic) / Solver.set_state(W)
+    PeriodicBC(W)          -> Solver.periodic_bc()
+    timestep(W)            -> Solver.timestep(cfl)            (sets Solver.dt)
+    evolveEulerian(W, dev) -> Solver.evolve_eulerian(nsteps)  (one step each)
+    PrintStats(W, splines) -> Solver.print_stats()            (stats dict + line)
+    Evolve(W, comm3d)      -> Solver.evolve(t_end)
+
+The state lives on the GPU between calls; ``get_state`` copies it back in the
+reference layout (five padded FP64 planes, Is(j, i, k)).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .ic import DhitIC, conservative_padded, dhit
+
+
+@dataclass
+class GasParams:
+    gamma: float = 1.4
+    R: float = 287.0          # weno/src/param.h:18
+    Pr: float = 0.7           # applied in the slope solver as GKMflux.cpp:289 does
+    mu0: float = 1e-3
+    T0: float = 1.0
+    visc_law: str = "power"   # "power" (mu ~ T^visc_exp) or "sutherland"
+    visc_exp: float = 0.76
+    recon: str = "symoo"      # "symoo" (reference) or "js"
+    energy_recon: str = "T"   # "T" (CPU path) or "E" (GPU path)
+    zbug_compat: bool = True  # reference Z-sweep tangential pairing
+    precision: str = "fp64"   # "fp64" or "mixed"
+    K: int | None = None      # internal DOF; default int((5 - 3 gamma)/(gamma - 1))
+
+    def dof(self) -> int:
+        if self.K is not None:
+            return int(self.K)
+        return int((5.0 - 3.0 * self.gamma) / (self.gamma - 1.0))  # FlowField.cpp:41
+
+    def to_c(self) -> _abi.GkmParams:
+        return _abi.GkmParams(
+            gamma=self.gamma, R=self.R, Pr=self.Pr, mu0=self.mu0, T0=self.T0, visc_exp=self.visc_exp,
+            K=self.dof(), visc_law=_abi.VISC[self.visc_law], recon=_abi.RECON[self.recon],
+            energy_recon=_abi.ENERGY[self.energy_recon], zbug_compat=int(bool(self.zbug_compat)),
+            precision=_abi.PRECISION[self.precision])
+
+
+@dataclass
+class RunInfo:
+    t: float = 0.0
+    t0: float = 1.0      # eddy turnover time used to print t/t0
+    dt: float = 0.0
+    steps: int = 0
+    history: list = field(default_factory=list)
+
+
+class Solver:
+    """One periodic box (nx, ny, nz interior cells, 3 ghost layers) on one GPU."""
+
+    def __init__(self, nx: int, ny: int | None = None, nz: int | None = None, dx: float | None = None,
+                 gas: GasParams | None = None):
+        self.nx = int(nx)
+        self.ny = int(nx if ny is None else ny)
+        self.nz = int(nx if nz is None else nz)
+        # dx = 2 pi / nc (FlowField.cpp:45); for uneven boxes the x extent sets it
+        self.dx = float(dx) if dx is not None else 2.0 * math.pi / self.nx
+        self.gas = gas or GasParams()
+        self.shape = (self.ny + 6, self.nx + 6, self.nz + 6)
+        self.ncell = int(np.prod(self.shape))
+        self.info = RunInfo()
+        self._L = _abi.load()
+        self._h = C.c_void_p()
+        grid = _abi.GkmGrid(self.nx, self.ny, self.nz, 3, self.dx)
+        self._params = self.gas.to_c()
+        _abi.check(self._L.gkm_create(C.byref(grid), C.byref(self._params), C.byref(self._h)))
+
+    # -- construction -----------------------------------------------------------------
+    @classmethod
+    def from_dhit(cls, n: int, k0: float = 4.0, Mt: float = 0.3, Re: float = 30.0, seed: int = 1234,
+                  precision: str = "fp64", visc_law: str = "power", recon: str = "symoo",
+                  energy_recon: str = "T", zbug_compat: bool = True, Pr: float = 0.7, gamma: float = 1.4,
+                  R: float = 287.0, ny: int | None = None, nz: int | None = None) -> "Solver":
+        """Box of decaying isotropic turbulence with the init_BS.m recipe
+        (T0 = p0/(R rho0) as FlowField.cpp:46)."""
+        ic = dhit(n, k0=k0, Mt=Mt, Re=Re, seed=seed, gamma=gamma, ny=ny, nz=nz)
+        gas = GasParams(gamma=gamma, R=R, Pr=Pr, mu0=ic.mu0, T0=ic.p0 / (R * ic.rho0), visc_law=visc_law,
+                        recon=recon, energy_recon=energy_recon, zbug_compat=zbug_compat, precision=precision)
+        s = cls(n, ny, nz, gas=gas)
+        s.initialize_dhit(ic)
+        return s
+
+    def initialize_dhit(self, ic: DhitIC) -> None:
+        """Initialize_DHIT (FlowField.cpp:59-112) followed by the first halo fill."""
+        self.ic = ic
+        self.info.t0 = ic.t0
+        self.set_state(conservative_padded(ic))
+        self.periodic_bc()
+
+    # -- state ------------------------------------------------------------------------
+    def set_state(self, W) -> None:
+        W = [np.ascontiguousarray(np.asarray(w, dtype=np.float64).reshape(-1)) for w in W]
+        if len(W) != 5 or any(w.size != self.ncell for w in W):
+            raise ValueError(f"expected 5 planes of {self.ncell} values")
+        _abi.check(self._L.gkm_set_state(self._h, _abi.ptr_array(W)))
+
+    def get_state(self):
+        W = [np.empty(self.ncell) for _ in range(5)]
+        _abi.check(self._L.gkm_get_state(self._h, _abi.ptr_array(W)))
+        return W
+
+    def periodic_bc(self) -> None:
+        _abi.check(self._L.gkm_fill_ghosts(self._h))
+
+    # -- stepping ---------------------------------------------------------------------
+    def timestep(self, cfl: float = 0.02) -> float:
+        """dt = cfl dx / max(|u| + c) (update.cpp:1057-1079); cfl default as param.h:17."""
+        dt = C.c_double()
+        _abi.check(self._L.gkm_compute_dt(self._h, float(cfl), C.byref(dt)))
+        self.info.dt = dt.value
+        return dt.value
+
+    def evolve_eulerian(self, nsteps: int = 1, dt: float | None = None) -> None:
+        dt = self.info.dt if dt is None else dt
+        if not dt > 0:
+            raise ValueError("dt is not set: call timestep() or pass dt")
+        _abi.check(self._L.gkm_step(self._h, float(dt), int(nsteps)))
+        self.info.t += nsteps * dt
+        self.info.steps += nsteps
+
+    def sync(self) -> None:
+        _abi.check(self._L.gkm_sync(self._h))
+
+    def stats(self) -> dict:
+        s = _abi.GkmStats()
+        _abi.check(self._L.gkm_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def print_stats(self, Mt: float | None = None, p0: float | None = None, T0: float | None = None,
+                    rho0: float = 1.0, file=None) -> dict:
+        """Statistics with PrintStats' columns (update.cpp:748-781):
+        t/t0 TKE Diss SKEWu SKEWv SKEWw KURTu KURTv KURTw Prms Trms Vrms.
+        Gradients are 2nd-order central differences."""
+        s = self.stats()
+        ic = getattr(self, "ic", None)
+        Mt = Mt if Mt is not None else (ic.Mt if ic else s["mt"])
+        p0 = p0 if p0 is not None else (ic.p0 if ic else s["p_mean"])
+        T0 = T0 if T0 is not None else self.gas.T0
+        g = self.gas.gamma
+        prms = s["p_rms"] / (g * p0 * Mt * Mt)
+        trms = s["T_rms"] / ((g - 1) * T0 * Mt * Mt)
+        vrms = s["v_rms"] / (Mt * Mt / rho0)
+        cols = [self.info.t / self.info.t0, s["tke"], s["diss_ref"], s["skew_u"], s["skew_v"], s["skew_w"],
+                s["kurt_u"], s["kurt_v"], s["kurt_w"], prms, trms, vrms]
+        line = "  ".join(f"{x:.8e}" for x in cols)
+        if file is not None:
+            print(line, file=file)
+        s["line"] = line
+        return s
+
+    def evolve(self, t_end: float, cfl: float = 0.02, stats_every: int = 1, recompute_dt: bool = False,
+               callback=None) -> list:
+        """The Evolve loop (update.cpp:106-150): dt fixed from the initial
+        state unless recompute_dt; statistics before each step; t += dt."""
+        self.periodic_bc()
+        if self.info.dt <= 0 or recompute_dt:
+            self.timestep(cfl)
+        while self.info.t <= t_end:
+            if self.info.steps % stats_every == 0:
+                s = self.print_stats()
+                self.info.history.append((self.info.t, s))
+                if callback:
+                    callback(self, s)
+            self.evolve_eulerian(1)
+            if recompute_dt:
+                self.timestep(cfl)
+        s = self.print_stats()
+        self.info.history.append((self.info.t, s))
+        return self.info.history
+
+    # -- timing -----------------------------------------------------------------------
+    def set_timing(self, on: bool = True) -> None:
+        _abi.check(self._L.gkm_set_timing(self._h, int(on)))
+
+    def last_timing(self):
+        ms = (C.c_double * 4)()
+        n = C.c_int()
+        _abi.check(self._L.gkm_last_timing(self._h, ms, C.byref(n)))
+        return {"flux_ms": ms[0], "recon_ms": ms[1], "ghost_ms": ms[2], "update_ms": ms[3], "steps": n.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.gkm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def flux_points(gas: GasParams, rec: np.ndarray, dx: float) -> np.ndarray:
+    """Device BGK flux on n records (n, 52) -> (n, 5) (gkm_flux_points)."""
+    L = _abi.load()
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    out = np.zeros((rec.shape[0], 5))
+    p = gas.to_c()
+    _abi.check(L.gkm_flux_points(C.byref(p), rec.shape[0], rec.ctypes.data_as(_abi.DP), float(dx),
+                                 out.ctypes.data_as(_abi.DP)))
+    return out
+
+
+def weno_points(gas: GasParams, rec: np.ndarray) -> np.ndarray:
+    """Device WENO on n records (n, 6, 5) -> (n, 10) = Wl[5] Wr[5] (gkm_weno_points)."""
+    L = _abi.load()
+    rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 30)
+    out = np.zeros((rec.shape[0], 10))
+    p = gas.to_c()
+    _abi.check(L.gkm_weno_points(C.byref(p), rec.shape[0], rec.ctypes.data_as(_abi.DP),
+                                 out.ctypes.data_as(_abi.DP)))
+    return out

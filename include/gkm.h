@@ -1,0 +1,167 @@
+/*
+ * gkm.h — C ABI of the MI355X GKM-DNS time step (libgkm.so).
+ *
+ * One handle owns one periodic box (or one slab) of the conservative state
+ * W = (rho, rho*u, rho*v, rho*w, rho*E) on the GPU, with 3 ghost layers per
+ * side.  Host arrays exchanged with the library use the reference layout:
+ * five separate FP64 arrays, each (ny+6)*(nx+6)*(nz+6) long, index
+ *     Is(j, i, k) = j*(nx+6)*(nz+6) + i*(nz+6) + k
+ * with j along y (slowest), i along x, k along z (unit stride)
+ * (weno/src/param.h:37).  Velocity components follow the axes: W1 = rho*u
+ * (x = axis 1), W2 = rho*v (y = axis 0), W3 = rho*w (z = axis 2).
+ *
+ * Every function returns 0 (GKM_OK) or a negative status; the message of the
+ * last failure on the calling thread is returned by gkm_last_error().
+ * No C++ exception crosses this boundary.  Calls on one handle are not
+ * thread-safe; separate handles may be used from separate threads.
+ */
+#ifndef GKM_H
+#define GKM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define GKM_OK             0
+#define GKM_E_ARG         -1   /* invalid argument (null pointer, bad size)  */
+#define GKM_E_HIP         -2   /* a HIP runtime call failed                    */
+#define GKM_E_NONPHYSICAL -3   /* rho <= 0, p <= 0 or non-finite state         */
+#define GKM_E_NOMEM       -4   /* device allocation failed                     */
+#define GKM_E_UNSUPPORTED -5   /* option not available in this build           */
+
+/* ---- option values -------------------------------------------------------- */
+#define GKM_VISC_SUTHERLAND 0  /* mu0 (T/T0)^1.5 (T0+110.4)/(T+110.4), weno/src/update.cpp:263 */
+#define GKM_VISC_POWER      1  /* mu0 (T/T0)^visc_exp, GKM/CUDA GKM/src/GPUupdate.cu:1327    */
+
+#define GKM_RECON_SYMOO 0      /* WENO-SYMOO, weno/src/update.cpp:502-625          */
+#define GKM_RECON_JS    1      /* WENO-JS 5th order, weno/src/update.cpp:1421-1513 */
+
+#define GKM_ENERGY_T 0         /* reconstruct T, rebuild rho*E (CPU path, update.cpp:560-619) */
+#define GKM_ENERGY_E 1         /* reconstruct rho*E directly (GPU path, GPUupdate.cu:546)     */
+
+#define GKM_PREC_FP64  0       /* FP64 everywhere (drop-in form)                               */
+#define GKM_PREC_MIXED 1       /* FP32 reconstruction + flux, FP32 hi/lo compensated state     */
+
+/* Grid of one box.  The reference uses one spacing for all axes
+ * (dx = 2*pi/nc, weno/src/FlowField.cpp:45). */
+typedef struct gkm_grid {
+    int    nx, ny, nz;   /* interior cells along x, y, z                      */
+    int    ghost;        /* ghost layers per side; must be 3                  */
+    double dx;           /* cell size                                         */
+} gkm_grid;
+
+/* Gas and scheme parameters.  In the reference these are compile-time
+ * macros and globals (weno/src/param.h:11-23, FlowField.cpp:40-49). */
+typedef struct gkm_params {
+    double gamma;        /* ratio of specific heats (gam)                     */
+    double R;            /* gas constant in T = p/(R rho) (param.h:18: 287)   */
+    double Pr;           /* Prandtl number applied in the slope solver on a[4]
+                            (GKMflux.cpp:289 uses 0.7; GPUupdate.cu:839 uses 1) */
+    double mu0;          /* reference viscosity                               */
+    double T0;           /* reference temperature                             */
+    double visc_exp;     /* exponent of GKM_VISC_POWER (0.76)                 */
+    int    K;            /* internal degrees of freedom, (5-3 gamma)/(gamma-1) */
+    int    visc_law;     /* GKM_VISC_*                                        */
+    int    recon;        /* GKM_RECON_*                                       */
+    int    energy_recon; /* GKM_ENERGY_*                                      */
+    int    zbug_compat;  /* 1: Z sweep pairs the y-slot slope with d/dx and the
+                            z-slot slope with d/dy, as update.cpp:380-423 does;
+                            0: physically consistent pairing                  */
+    int    precision;    /* GKM_PREC_*                                        */
+} gkm_params;
+
+/* Statistics of the interior cells, all accumulated in FP64.
+ * Means are over the nx*ny*nz interior cells. */
+typedef struct gkm_stats_s {
+    double tke;        /* <rho u_i u_i>   (update.cpp PrintStats "TKE")        */
+    double ek;         /* 0.5 <rho u_i u_i>                                    */
+    double q2;         /* <u_i u_i>                                            */
+    double mt;         /* sqrt(<u_i u_i>) / <c>, c = sqrt(gamma p / rho)       */
+    double rho_mean, rho_rms;
+    double p_mean, p_rms;
+    double T_mean, T_rms;
+    double v_mean, v_rms;       /* specific volume 1/rho                       */
+    double c_mean;
+    double umax_plus_c;         /* max(|u| + c), the CFL signal speed          */
+    /* velocity-gradient statistics, 2nd-order central differences */
+    double dudx2, dudx3, dudx4; /* <(du/dx)^n>                                 */
+    double dvdy2, dvdy3, dvdy4;
+    double dwdz2, dwdz3, dwdz4;
+    double skew_u, skew_v, skew_w;
+    double kurt_u, kurt_v, kurt_w;
+    double enstrophy;           /* <|curl u|^2>                                */
+    double dil2;                /* <(div u)^2>                                 */
+    double diss_sol;            /* <mu |curl u|^2>                             */
+    double diss_dil;            /* 4/3 <mu (div u)^2>                          */
+    double diss_ref;            /* <mu du_i/dx_j du_i/dx_j> (PrintStats Diss)  */
+    double sijsij;              /* <S_ij S_ij> as PrintStats forms it          */
+    double mass, mom_x, mom_y, mom_z, energy; /* interior sums of W0..W4       */
+} gkm_stats_t;
+
+typedef struct gkm_ctx gkm_t;
+
+/* Create a handle on the current HIP device and allocate the device state.
+ * Replaces GetParams/segment/DevAlloc (weno/src/FlowField.cpp:9-57,
+ * FlowField.cpp:278-288, GPUAlloc.cu:3-17). */
+int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out);
+int gkm_destroy(gkm_t* h);
+
+/* Copy the padded host state in / out (bit-exact round trip for FP64; the
+ * mixed form splits each value into an FP32 hi/lo pair and rebuilds it).
+ * Replaces Initialize_DHIT's writes into W and the H2D/D2H copies of
+ * evolve() (FlowField.cpp:59-112, GPUupdate.cu:74-80,186-190). */
+int gkm_set_state(gkm_t* h, const double* const W[5]);
+int gkm_get_state(gkm_t* h, double* const W[5]);
+
+/* 3-deep periodic ghost fill on the device (replaces PeriodicBC and the
+ * one-rank HaloTransfer, FlowField.cpp:115-275, update.cpp:894-1054). */
+int gkm_fill_ghosts(gkm_t* h);
+
+/* Advance nsteps steps of size dt: per step, ghost fill, X/Y/Z WENO + BGK
+ * flux sweeps and the conservative update; a final ghost fill leaves the
+ * state as the reference leaves it after HaloTransfer.  Replaces
+ * evolve(W, dev, dt) / evolveEulerian (GPUupdate.cu:65-196,
+ * update.cpp:180-199).  Asynchronous on the handle's stream. */
+int gkm_step(gkm_t* h, double dt, int nsteps);
+
+/* dt = cfl*dx / max_interior(|u| + sqrt(gamma p/rho)), FP64 device
+ * reduction (replaces timestep, update.cpp:1057-1079). */
+int gkm_compute_dt(gkm_t* h, double cfl, double* dt);
+
+/* FP64 device reductions of the interior (replaces the sums of PrintStats,
+ * update.cpp:638-784, and the GKM on-device TKE/PRMS, GPUupdate.cu:425-483). */
+int gkm_stats(gkm_t* h, gkm_stats_t* out);
+
+/* Wait for all work queued on the handle's stream. */
+int gkm_sync(gkm_t* h);
+
+/* Device time of the last gkm_step call, from HIP events recorded on the
+ * handle's stream around each launch (first 64 steps of the call):
+ * ms[0] BGK flux kernels, ms[1] WENO reconstruction kernels, ms[2] ghost
+ * fills, ms[3] conservative updates; -1 where nothing was recorded.
+ * *nsteps receives the number of timed steps.  Timing is off by default. */
+int gkm_set_timing(gkm_t* h, int enabled);
+int gkm_last_timing(gkm_t* h, double ms[4], int* nsteps);
+
+/* Point-wise entry points used to check the device arithmetic against the
+ * reference on sampled interfaces.  Replace flux() (GKMflux.cpp:11-215) and
+ * ApplyWENO / d_ApplyWENO (update.cpp:502-625, GPUupdate.cu:917-977).
+ * Inputs are host arrays of n records; the library copies them to the
+ * device, runs the device code and copies the result back.
+ *   flux:  in[n*52] = WL[5] WR[5] Wl[5] Wr[5] DWxl[5] DWxr[5] DWyl[5] DWyr[5]
+ *                     DWzl[5] DWzr[5] tau dt, with dx given;  F[n*5]
+ *   weno:  in[n*30] = six cells x 5 fields (LLL, LL, L, R, RR, RRR);
+ *          out[n*10] = Wl[5] Wr[5]                                         */
+int gkm_flux_points(const gkm_params* p, int n, const double* in, double dx, double* F);
+int gkm_weno_points(const gkm_params* p, int n, const double* in, double* out);
+
+const char* gkm_last_error(void);
+
+/* Library version string, e.g. "gkm-mi355x 0.1 gfx950". */
+const char* gkm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GKM_H */
