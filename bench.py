@@ -1,0 +1,186 @@
+"""Throughput of the GKM-DNS step on one MI355X.
+
+Workload: 256^3 periodic box of decaying isotropic turbulence (init_BS.m
+recipe, k0 = 4, Mt = 0.5, Re = 70, gamma = 1.4, Pr = 0.7, mu ~ T^0.76),
+WENO-JS reconstruction with temperature recovery, BGK flux, single-stage
+update, 3-deep periodic ghost fill every step; dt fixed from CFL 0.5 at t = 0
+(the reference computes dt once, weno/src/update.cpp:112).
+
+One "step" = ghost fill + X/Y/Z reconstruction and flux sweeps + update of the
+whole box.  The headline value is the mixed form (FP32 reconstruction/flux,
+FP32 hi/lo state); the FP64 drop-in form is timed in the same run on the same
+initial condition, and the relative E_k / Mt error of the mixed run against the
+FP64 run after the same steps is reported.
+
+The CPU baseline is the FP64 oracle (the restated reference CPU step) on a
+bounded sample: a 64^3 box of the same recipe for a few steps, single-threaded
+like the reference, plus an OpenMP run over the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "gkm-dns_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_VEC_TFLOPS = 78.6         # MI355X vector FP64
+FP32_VEC_TFLOPS = 157.3        # MI355X vector FP32
+ALG_BYTES_PER_CELL = 80.0      # 5 state words read + 5 written per cell-update, 8 B each
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="box edge (cells)")
+    ap.add_argument("--mt", type=float, default=0.5)
+    ap.add_argument("--re", type=float, default=70.0)
+    ap.add_argument("--recon", default="js", choices=["js", "symoo"])
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run")
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    return ap.parse_args()
+
+
+def time_solver(s, dt, steps, warmup):
+    s.evolve_eulerian(warmup, dt=dt)
+    s.sync()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.evolve_eulerian(steps, dt=dt)
+    s.sync()
+    t1 = time.perf_counter()
+    tim = s.last_timing()
+    s.set_timing(False)
+    return (t1 - t0), tim
+
+
+def cpu_baseline(args, gas_kw):
+    """FP64 oracle on a bounded sample of the workload (same recipe, smaller box)."""
+    import numpy as np  # noqa: F401
+
+    from gkm_amd.ic import conservative_padded, dhit
+    from oracle import oracle as O
+
+    n = args.cpu_n
+    ic = dhit(n, k0=4, Mt=args.mt, Re=args.re, seed=1234)
+    W = conservative_padded(ic)
+    g = O.make_grid(n, n, n, 2 * math.pi / n)
+    P = O.make_params(gamma=1.4, R=287.0, Pr=0.7, mu0=ic.mu0, T0=ic.p0 / 287.0, visc_exp=0.76, K=2, visc_law=1,
+                      recon=1 if args.recon == "js" else 0, energy_recon=0, zbug_compat=1)
+    O.fill_ghosts(g, W)
+    dt = O.timestep(g, P, W, args.cfl)
+    t0 = time.perf_counter()
+    O.step(g, P, W, dt, args.cpu_steps, 1)
+    t1 = time.perf_counter()
+    one = n ** 3 * args.cpu_steps / (t1 - t0)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    O.step(g, P, W, dt, args.cpu_steps, threads)
+    t1 = time.perf_counter()
+    many = n ** 3 * args.cpu_steps / (t1 - t0)
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": one, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, {n}^3 box of the same recipe, {args.cpu_steps} steps, 1 thread",
+            "omp_value": many, "omp_threads": threads, "cpu_model": cpu_model}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": f"this build runs on one GPU; --gpus {args.gpus} is not supported",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import gkm_amd
+
+    n = args.n
+    cells = float(n) ** 3
+    kw = dict(visc_law="power", recon=args.recon, energy_recon="T")
+    t_ic = time.perf_counter()
+    ic = gkm_amd.dhit(n, k0=4, Mt=args.mt, Re=args.re, seed=1234)
+    print(f"[bench] initial condition {n}^3 in {time.perf_counter() - t_ic:.1f} s", file=sys.stderr, flush=True)
+    mixed = gkm_amd.Solver.from_ic(ic, precision="mixed", **kw)
+    dt = mixed.timestep(args.cfl)
+    sec_m, tim_m = time_solver(mixed, dt, args.steps, args.warmup)
+    print(f"[bench] mixed: {1e3 * sec_m / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+    stats_m = mixed.stats()
+    mixed.close()
+    rate_m = cells * args.steps / sec_m
+
+    res_fp64 = None
+    if not args.no_fp64:
+        fp64 = gkm_amd.Solver.from_ic(ic, precision="fp64", **kw)
+        sec_d, tim_d = time_solver(fp64, dt, args.steps, args.warmup)
+        print(f"[bench] fp64: {1e3 * sec_d / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+        stats_d = fp64.stats()
+        fp64.close()
+        rate_d = cells * args.steps / sec_d
+        res_fp64 = dict(value=rate_d, ms_per_step=1e3 * sec_d / args.steps,
+                        flux_kernel_ms=tim_d["flux_ms"] / (3 * tim_d["steps"]),
+                        breakdown_ms_per_step={k: v / tim_d["steps"] for k, v in tim_d.items() if k != "steps"})
+        ek_err = abs(stats_m["ek"] - stats_d["ek"]) / abs(stats_d["ek"])
+        mt_err = abs(stats_m["mt"] - stats_d["mt"]) / abs(stats_d["mt"])
+    else:
+        ek_err = mt_err = None
+
+    # dominant kernel: the BGK flux sweep (one launch per direction per step)
+    flux_ms = tim_m["flux_ms"] / (3 * tim_m["steps"])
+    alg_bytes_launch = ALG_BYTES_PER_CELL * cells / 3.0
+    achieved = alg_bytes_launch / (flux_ms * 1e-3) / 1e9
+    step_hbm = ALG_BYTES_PER_CELL * rate_m / 1e9
+
+    out = {
+        "metric": "cell-updates/s, mixed vs FP64, 256^3 on 1 MI355X; % HBM roofline; E_k err",
+        "value": rate_m,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec_m / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (reconstruction, flux) + f32 hi/lo state + f64 T recovery/reductions",
+        "data": "synthetic: init_BS.m spectrum, seeded",
+        "config": {"workload": f"decaying isotropic turbulence {n}^3, Mt={args.mt}, Re={args.re}, k0=4, "
+                               f"WENO-{args.recon.upper()} + BGK, CFL {args.cfl}, mixed precision",
+                   "box": [n, n, n], "ghost": 3, "dt": dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_flux (BGK flux sweep)", "kernel_ms": flux_ms,
+                     "alg_bytes_per_launch": alg_bytes_launch,
+                     "step_frac": step_hbm / HBM_PEAK_GBS},
+        "breakdown_ms_per_step": {k: v / tim_m["steps"] for k, v in tim_m.items() if k != "steps"},
+        "fp64": res_fp64,
+        "mixed_over_fp64": (rate_m / res_fp64["value"]) if res_fp64 else None,
+        "ek_rel_err": ek_err,
+        "mt_rel_err": mt_err,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, kw)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -86,15 +86,22 @@ class Solver:
     # -- construction -----------------------------------------------------------------
     @classmethod
     def from_dhit(cls, n: int, k0: float = 4.0, Mt: float = 0.3, Re: float = 30.0, seed: int = 1234,
-                  precision: str = "fp64", visc_law: str = "power", recon: str = "symoo",
-                  energy_recon: str = "T", zbug_compat: bool = True, Pr: float = 0.7, gamma: float = 1.4,
-                  R: float = 287.0, ny: int | None = None, nz: int | None = None) -> "Solver":
-        """Box of decaying isotropic turbulence with the init_BS.m recipe
-        (T0 = p0/(R rho0) as FlowField.cpp:46)."""
+                  ny: int | None = None, nz: int | None = None, gamma: float = 1.4, **gas_kw) -> "Solver":
+        """Box of decaying isotropic turbulence with the init_BS.m recipe."""
         ic = dhit(n, k0=k0, Mt=Mt, Re=Re, seed=seed, gamma=gamma, ny=ny, nz=nz)
-        gas = GasParams(gamma=gamma, R=R, Pr=Pr, mu0=ic.mu0, T0=ic.p0 / (R * ic.rho0), visc_law=visc_law,
-                        recon=recon, energy_recon=energy_recon, zbug_compat=zbug_compat, precision=precision)
-        s = cls(n, ny, nz, gas=gas)
+        return cls.from_ic(ic, **gas_kw)
+
+    @classmethod
+    def from_ic(cls, ic: DhitIC, precision: str = "fp64", visc_law: str = "power", recon: str = "symoo",
+                energy_recon: str = "T", zbug_compat: bool = True, Pr: float = 0.7, R: float = 287.0,
+                visc_exp: float = 0.76) -> "Solver":
+        """Solver on the IC's grid with mu0 from the IC and T0 = p0/(R rho0)
+        (weno/src/FlowField.cpp:46)."""
+        ny, nx, nz = ic.u.shape
+        gas = GasParams(gamma=ic.gamma, R=R, Pr=Pr, mu0=ic.mu0, T0=ic.p0 / (R * ic.rho0), visc_law=visc_law,
+                        visc_exp=visc_exp, recon=recon, energy_recon=energy_recon, zbug_compat=zbug_compat,
+                        precision=precision)
+        s = cls(nx, ny, nz, gas=gas)
         s.initialize_dhit(ic)
         return s
 
