@@ -93,7 +93,7 @@ int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
     const bool fp64 = params->precision == GKM_PREC_FP64;
     const size_t plane = up(n * (fp64 ? sizeof(double) : sizeof(float)));
-    const int nplanes = fp64 ? (5 + 10 + 15) : (10 + 10 + 15);
+    const int nplanes = fp64 ? (5 + 10 + 15 + 2) : (10 + 10 + 15 + 2);
     h->red_blocks = 2048;
     const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
     h->arena_bytes = plane * nplanes + red;
@@ -113,11 +113,15 @@ int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
         for (int q = 0; q < 5; q++) h->Wd[q] = (double*)take(plane);
         for (int q = 0; q < 10; q++) h->WId[q] = (double*)take(plane);
         for (int q = 0; q < 15; q++) h->Fd[q] = (double*)take(plane);
+        h->Td = (double*)take(plane);
+        h->taud = (double*)take(plane);
     } else {
         for (int q = 0; q < 5; q++) h->Whi[q] = (float*)take(plane);
         for (int q = 0; q < 5; q++) h->Wlo[q] = (float*)take(plane);
         for (int q = 0; q < 10; q++) h->WIf[q] = (float*)take(plane);
         for (int q = 0; q < 15; q++) h->Ff[q] = (float*)take(plane);
+        h->Tf = (float*)take(plane);
+        h->tauf = (float*)take(plane);
     }
     h->red_part = (double*)take(up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA));
     h->red_out = (double*)take(up(sizeof(double) * 64));
@@ -188,7 +192,7 @@ int gkm_get_state(gkm_t* hh, double* const W[5]) {
 int gkm_fill_ghosts(gkm_t* hh) {
     Ctx* h = (Ctx*)hh;
     if (!h) return fail(GKM_E_ARG, "null handle");
-    gkm::launch_ghost(h);
+    gkm::launch_ghost(h, nullptr);
     HIP_TRY(hipGetLastError());
     return GKM_OK;
 }
@@ -208,7 +212,7 @@ int gkm_step(gkm_t* hh, double dt, int nsteps) {
         gkm::launch_step(h, dt, timed, s);
         if (timed) h->timed_steps = s + 1;
     }
-    gkm::launch_ghost(h);
+    gkm::launch_ghost(h, nullptr);
     HIP_TRY(hipGetLastError());
     return GKM_OK;
 }
@@ -271,7 +275,7 @@ int gkm_compute_dt(gkm_t* hh, double cfl, double* dt) {
 int gkm_stats(gkm_t* hh, gkm_stats_t* out) {
     Ctx* h = (Ctx*)hh;
     if (!h || !out) return fail(GKM_E_ARG, "null argument");
-    gkm::launch_ghost(h);
+    gkm::launch_ghost(h, nullptr);
     double a[gkm::kStatA];
     HIP_TRY(gkm::launch_stats_a(h, a));
     const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;

@@ -20,19 +20,30 @@
 namespace gkm {
 
 template <class T> struct M;
+// FP64: the ROCm device library for erfc/exp/pow; reciprocals from v_rcp_f64
+// refined by two Newton steps (within an ulp of 1/x for the normal-range
+// values that occur here; the IEEE division sequence is about twice as long).
 template <> struct M<double> {
     __device__ static __forceinline__ double erfc_(double x) { return ::erfc(x); }
     __device__ static __forceinline__ double exp_(double x) { return ::exp(x); }
     __device__ static __forceinline__ double sqrt_(double x) { return ::sqrt(x); }
     __device__ static __forceinline__ double pow_(double x, double y) { return ::pow(x, y); }
-    __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+    __device__ static __forceinline__ double rcp(double x) {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-x, r, 1.0);
+        return fma(r, e, r);
+    }
 };
+// FP32 (mixed form): hardware reciprocal, exponential and square root
+// (v_rcp_f32, v_exp_f32, v_sqrt_f32, about 1 ulp); erfc from the device library.
 template <> struct M<float> {
     __device__ static __forceinline__ float erfc_(float x) { return ::erfcf(x); }
-    __device__ static __forceinline__ float exp_(float x) { return ::expf(x); }
-    __device__ static __forceinline__ float sqrt_(float x) { return ::sqrtf(x); }
+    __device__ static __forceinline__ float exp_(float x) { return __expf(x); }
+    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ static __forceinline__ float pow_(float x, float y) { return ::powf(x, y); }
-    __device__ static __forceinline__ float rcp(float x) { return 1.0f / x; }
+    __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
 // ---------------------------------------------------------------------------
@@ -116,6 +127,7 @@ template <class T> struct Gas {
     T gam;     // ratio of specific heats
     T invPr;   // 1/Pr applied on a[4] in the slope solver (GKMflux.cpp:289)
     T kp3;     // K + 3
+    T ikp3;    // 1/(K + 3)
     T K;       // internal degrees of freedom
 };
 
@@ -162,16 +174,17 @@ __device__ __forceinline__ void mom_a_acc(T acc[5], T s, const T* n, const T* v,
     acc[4] += s * m4;
 }
 
-// Closed-form inverse of the Maxwellian moment matrix: b -> a (GKMflux.cpp:280-296).
+// Closed-form inverse of the Maxwellian moment matrix: b -> a (GKMflux.cpp:280-296);
+// h = 1/(2 lam).
 template <class T>
-__device__ __forceinline__ void slope_solve(const Gas<T>& g, const T b[5], const T U[3], T lam, T a[5]) {
+__device__ __forceinline__ void slope_solve(const Gas<T>& g, const T b[5], const T U[3], T lam, T h, T a[5]) {
     const T uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
     const T R2 = b[1] - U[0] * b[0];
     const T R3 = b[2] - U[1] * b[0];
     const T R4 = b[3] - U[2] * b[0];
-    const T q = g.kp3 * M<T>::rcp(2 * lam);  // (K+3)/(2 lam)
+    const T q = g.kp3 * h;  // (K+3)/(2 lam)
     const T R5 = 2 * b[4] - b[0] * (uu + q);
-    a[4] = g.invPr * (R5 - 2 * U[0] * R2 - 2 * U[1] * R3 - 2 * U[2] * R4) * (4 * lam * lam) * M<T>::rcp(g.kp3);
+    a[4] = g.invPr * (R5 - 2 * U[0] * R2 - 2 * U[1] * R3 - 2 * U[2] * R4) * (4 * lam * lam) * g.ikp3;
     a[3] = 2 * lam * R4 - U[2] * a[4];
     a[2] = 2 * lam * R3 - U[1] * a[4];
     a[1] = 2 * lam * R2 - U[0] * a[4];
@@ -233,20 +246,20 @@ __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, co
     T b[5], ax[5], ay[5], az[5], A[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWx[q] * ri;
-    slope_solve(g, b, s.U, s.lam, ax);
+    slope_solve(g, b, s.U, s.lam, s.h, ax);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWy[q] * ri;
-    slope_solve(g, b, s.U, s.lam, ay);
+    slope_solve(g, b, s.U, s.lam, s.h, ay);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWz[q] * ri;
-    slope_solve(g, b, s.U, s.lam, az);
+    slope_solve(g, b, s.U, s.lam, s.h, az);
     // time slope of this side: A = -(<u psi ax> + <v psi ay> + <w psi az>) over the full space
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = T(0);
     mom_a_acc<1, 0, 0>(b, T(-1), fn, fv, fw, s.e2, s.e4, ax);
     mom_a_acc<0, 1, 0>(b, T(-1), fn, fv, fw, s.e2, s.e4, ay);
     mom_a_acc<0, 0, 1>(b, T(-1), fn, fv, fw, s.e2, s.e4, az);
-    slope_solve(g, b, s.U, s.lam, A);
+    slope_solve(g, b, s.U, s.lam, s.h, A);
     // equilibrium tangential-slope sources: <psi ay>_half, <psi az>_half
     mom_a_acc<0, 0, 0>(by, T(1), hs, fv, fw, s.e2, s.e4, ay);
     mom_a_acc<0, 0, 0>(bz, T(1), hs, fv, fw, s.e2, s.e4, az);
@@ -353,12 +366,12 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     const T s2 = T(2) * M<T>::rcp(e.den * dx);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * s2;
-    slope_solve(g, b, e.U, e.lam, axl);
+    slope_solve(g, b, e.U, e.lam, e.h, axl);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * s2;
-    slope_solve(g, b, e.U, e.lam, axr);
-    slope_solve(g, by, e.U, e.lam, ay);
-    slope_solve(g, bz, e.U, e.lam, az);
+    slope_solve(g, b, e.U, e.lam, e.h, axr);
+    slope_solve(g, by, e.U, e.lam, e.h, ay);
+    slope_solve(g, bz, e.U, e.lam, e.h, az);
     const T de = e.den;
     {
         T m[5];
@@ -375,7 +388,7 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     const T ide = M<T>::rcp(de);
 #pragma unroll
     for (int q = 0; q < 5; q++) B[q] *= ide;
-    slope_solve(g, B, e.U, e.lam, A);
+    slope_solve(g, B, e.U, e.lam, e.h, A);
     {
         T m[5];
         mom<1, 0, 0>(m, fn, fv, fw, e.e2);

@@ -22,11 +22,15 @@ struct Ctx {
     double* Wd[5] = {};
     double* WId[10] = {};
     double* Fd[15] = {};
+    double* Td = nullptr;    // WENO temperature input per cell
+    double* taud = nullptr;  // left-cell collision time mu/p per cell
     // mixed form
     float* Whi[5] = {};
     float* Wlo[5] = {};
     float* WIf[10] = {};
     float* Ff[15] = {};
+    float* Tf = nullptr;
+    float* tauf = nullptr;
     // one device allocation holds every plane
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -42,7 +46,7 @@ struct Ctx {
 };
 
 StepPar step_par(const Ctx* h, double dt);
-void launch_ghost(Ctx* h);
+void launch_ghost(Ctx* h, const StepPar* par);  // par == nullptr: ghosts only
 void launch_step(Ctx* h, double dt, bool timed, int step_index);
 void launch_split(Ctx* h, int q, const double* dsrc);
 void launch_merge(Ctx* h, int q, double* ddst);

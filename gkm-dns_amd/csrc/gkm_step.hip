@@ -9,13 +9,24 @@ namespace gkm {
 
 // ---------------------------------------------------------------------------
 // Periodic ghost fill (replaces PeriodicBC / one-rank HaloTransfer,
-// weno/src/FlowField.cpp:115-275, update.cpp:894-1054).  Every ghost cell
-// takes the value of its periodic image in the interior, which is what the
-// three full-extent axis passes of the reference produce, corners included.
-// One thread per padded cell of the shell; interior threads exit.
+// weno/src/FlowField.cpp:115-275, update.cpp:894-1054) fused with the per-cell
+// primitives every sweep needs.  Every ghost cell takes the value of its
+// periodic image in the interior, which is what the three full-extent axis
+// passes of the reference produce, corners included.  Then, for every padded
+// cell, from the FP64 state:
+//   Tw  = (gam-1)(E/rho - |m|^2/(2 rho^2))/R   the WENO temperature input
+//         (update.cpp:560-565)
+//   tau = mu(Ti)/p, Ti = p/(R rho)            the left-cell collision time
+//         (update.cpp:262-264, 882-891)
+// so that the sweeps read them instead of recomputing them per interface.
 // ---------------------------------------------------------------------------
-template <int NPL, class T>
-__global__ void __launch_bounds__(256) k_ghost(Geo g, Planes10<T> f) {
+__device__ __forceinline__ double viscosity(const StepPar& par, double T) {
+    if (par.visc_law == 1) return par.mu0 * pow(T / par.T0, par.visc_exp);
+    return par.mu0 * (T / par.T0) * sqrt(T / par.T0) * (par.T0 + 110.4) / (T + 110.4);
+}
+
+template <int NPL, class T, class Real, class St>
+__global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Planes10<T> f, St st, Real* Tw, Real* tau, StepPar par) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)g.P0 * g.P1 * g.P2;
     if (t >= n) return;
@@ -24,13 +35,23 @@ __global__ void __launch_bounds__(256) k_ghost(Geo g, Planes10<T> f) {
     int i = (int)(r % g.P1);
     int j = (int)(r / g.P1);
     const bool gj = j < 3 || j >= g.ny + 3, gi = i < 3 || i >= g.nx + 3, gk = k < 3 || k >= g.nz + 3;
-    if (!(gj || gi || gk)) return;
-    int js = j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
-    int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
-    int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
-    long src = (long)js * g.s0 + (long)is * g.s1 + ks;
+    long src = t;
+    if (gj || gi || gk) {
+        int js = j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
+        int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
+        int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
+        src = (long)js * g.s0 + (long)is * g.s1 + ks;
 #pragma unroll
-    for (int q = 0; q < NPL; q++) f.p[q][t] = f.p[q][src];
+        for (int q = 0; q < NPL; q++) f.p[q][t] = f.p[q][src];
+    }
+    if (!Tw) return;
+    const double r0 = st.get(0, src), m1 = st.get(1, src), m2 = st.get(2, src), m3 = st.get(3, src),
+                 E = st.get(4, src);
+    Tw[t] = (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R);
+    const double u = m1 / r0, v = m2 / r0, w = m3 / r0;
+    const double P = (r0 * (par.gam - 1)) * (E / r0 - 0.5 * (u * u + v * v + w * w));
+    const double Ti = P / (par.R * r0);
+    tau[t] = (Real)(viscosity(par, Ti) / P);
 }
 
 // ---------------------------------------------------------------------------
@@ -41,7 +62,8 @@ __global__ void __launch_bounds__(256) k_ghost(Geo g, Planes10<T> f) {
 // reads the neighbours' values from here instead, which are the same numbers.
 // ---------------------------------------------------------------------------
 template <class Real, class St, int RECON>
-__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, long sn, St st, Planes10<Real> wi, StepPar par) {
+__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, long sn, St st, const Real* __restrict__ Tw,
+                                               Planes10<Real> wi, StepPar par) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
     const long c = box_index(g, b, t);
@@ -55,21 +77,15 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, long sn, St st, Pl
         weno_line<RECON>(v, eps, wl[q], wr[q]);
     }
     if (par.energy_recon == 0) {
-        // temperature reconstruction (update.cpp:560-621); the cell
-        // temperature is recovered from the conservative state in FP64
+        // temperature reconstruction (update.cpp:560-621), cell temperatures from the plane
         Real Tc[6];
 #pragma unroll
-        for (int s = 0; s < 6; s++) {
-            const long cc = c + (s - 2) * sn;
-            const double r0 = st.get(0, cc), m1 = st.get(1, cc), m2 = st.get(2, cc), m3 = st.get(3, cc),
-                         E = st.get(4, cc);
-            Tc[s] = (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R);
-        }
+        for (int s = 0; s < 6; s++) Tc[s] = Tw[c + (s - 2) * sn];
         Real Tl, Tr;
         weno_line<RECON>(Tc, eps, Tl, Tr);
         const Real cR = Real(par.R / (par.gam - 1.0));
-        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) / wl[0];
-        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) / wr[0];
+        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
+        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
     }
 #pragma unroll
     for (int q = 0; q < 5; q++) {
@@ -93,7 +109,8 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 // ---------------------------------------------------------------------------
 template <class Real, class St, int DIR>
 __global__ void __launch_bounds__(256) k_flux(Geo g, Box3 b, long sn, long st1, long st2, St st,
-                                              Planes10<Real> wi, Planes5<Real> F, StepPar par) {
+                                              const Real* __restrict__ tau, Planes10<Real> wi, Planes5<Real> F,
+                                              StepPar par) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
     const long c = box_index(g, b, t);
@@ -102,22 +119,11 @@ __global__ void __launch_bounds__(256) k_flux(Geo g, Box3 b, long sn, long st1, 
     gas.invPr = (Real)par.invPr;
     gas.K = (Real)par.K;
     gas.kp3 = (Real)(par.K + 3);
+    gas.ikp3 = (Real)(1.0 / (par.K + 3));
     const Real idx = Real(1.0 / par.dx);
 
-    // collision time of the left cell, from the FP64 state (update.cpp:262-264)
-    Real tau0;
-    {
-        const double den = st.get(0, c);
-        const double u = st.get(1, c) / den, v = st.get(2, c) / den, w = st.get(3, c) / den;
-        const double P = (den * (par.gam - 1)) * (st.get(4, c) / den - 0.5 * (u * u + v * v + w * w));
-        const double T = P / (par.R * den);
-        double mu;
-        if (par.visc_law == 1)
-            mu = par.mu0 * pow(T / par.T0, par.visc_exp);
-        else
-            mu = par.mu0 * (T / par.T0) * sqrt(T / par.T0) * (par.T0 + 110.4) / (T + 110.4);
-        tau0 = (Real)(mu / P);
-    }
+    // collision time mu/p of the left cell (update.cpp:262-264), from the plane
+    const Real tau0 = tau[c];
     Real WL[5], WR[5], Wl[5], Wr[5];
 #pragma unroll
     for (int s = 0; s < 5; s++) {
@@ -216,6 +222,7 @@ __global__ void k_flux_points(int n, const double* rec, double dx, double gam, d
     gas.invPr = (Real)invPr;
     gas.K = (Real)K;
     gas.kp3 = (Real)(K + 3);
+    gas.ikp3 = (Real)(1.0 / (K + 3));
     Real v[10][5];
     for (int a = 0; a < 10; a++)
         for (int q = 0; q < 5; q++) v[a][q] = (Real)x[a * 5 + q];
@@ -253,8 +260,8 @@ __global__ void k_weno_points(int n, const double* rec, double gam, double R, in
         Real Tl, Tr;
         weno_line<RECON>(Tc, eps, Tl, Tr);
         const Real cR = Real(R / (gam - 1.0));
-        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) / wl[0];
-        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) / wr[0];
+        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
+        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
     }
     for (int q = 0; q < 5; q++) {
         out[(long)t * 10 + q] = (double)wl[q];
@@ -304,8 +311,8 @@ static inline void tangential_axes(int dir, int zbug, int& t1, int& t2) {
 }
 
 template <class Real, class St>
-static void launch_sweep(Ctx* h, int dir, St st, Planes10<Real> wi, Planes5<Real> F, const StepPar& par,
-                         hipEvent_t* ev) {
+static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau, Planes10<Real> wi,
+                         Planes5<Real> F, const StepPar& par, hipEvent_t* ev) {
     const Geo& g = h->geo;
     const int an = normal_axis(dir);
     const long sn = axis_stride(g, an);
@@ -317,38 +324,43 @@ static void launch_sweep(Ctx* h, int dir, St st, Planes10<Real> wi, Planes5<Real
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->par.recon == GKM_RECON_JS)
         hipLaunchKernelGGL((k_recon<Real, St, 1>), dim3(blocks_for(be.count(), 256)), dim3(256), 0, h->stream, g, be,
-                           sn, st, wi, par);
+                           sn, st, Tw, wi, par);
     else
         hipLaunchKernelGGL((k_recon<Real, St, 0>), dim3(blocks_for(be.count(), 256)), dim3(256), 0, h->stream, g, be,
-                           sn, st, wi, par);
+                           sn, st, Tw, wi, par);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     const int bs = h->flux_block;
     if (dir == 0)
         hipLaunchKernelGGL((k_flux<Real, St, 0>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, wi, F, par);
+                           st1, st2, st, tau, wi, F, par);
     else if (dir == 1)
         hipLaunchKernelGGL((k_flux<Real, St, 1>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, wi, F, par);
+                           st1, st2, st, tau, wi, F, par);
     else
         hipLaunchKernelGGL((k_flux<Real, St, 2>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, wi, F, par);
+                           st1, st2, st, tau, wi, F, par);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
-void launch_ghost(Ctx* h) {
+void launch_ghost(Ctx* h, const StepPar* par) {
     const Geo& g = h->geo;
     long n = (long)g.P0 * g.P1 * g.P2;
+    StepPar p0 = par ? *par : StepPar{};
     if (h->par.precision == GKM_PREC_FP64) {
         Planes10<double> f{};
-        for (int q = 0; q < 5; q++) f.p[q] = h->Wd[q];
-        hipLaunchKernelGGL((k_ghost<5, double>), dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, g, f);
+        StateD st;
+        for (int q = 0; q < 5; q++) f.p[q] = st.w[q] = h->Wd[q];
+        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                           h->stream, g, f, st, par ? h->Td : nullptr, h->taud, p0);
     } else {
         Planes10<float> f{};
+        StateM st;
         for (int q = 0; q < 5; q++) {
-            f.p[q] = h->Whi[q];
-            f.p[5 + q] = h->Wlo[q];
+            f.p[q] = st.hi[q] = h->Whi[q];
+            f.p[5 + q] = st.lo[q] = h->Wlo[q];
         }
-        hipLaunchKernelGGL((k_ghost<10, float>), dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, g, f);
+        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                           h->stream, g, f, st, par ? h->Tf : nullptr, h->tauf, p0);
     }
 }
 
@@ -373,7 +385,7 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
     const StepPar par = step_par(h, dt);
     hipEvent_t* ev = timed ? h->ev_step + step_index * 12 : nullptr;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_ghost(h);
+    launch_ghost(h, &par);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     Box3 bi = interior_box(g);
     if (h->par.precision == GKM_PREC_FP64) {
@@ -384,7 +396,7 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
         for (int dir = 0; dir < 3; dir++) {
             Planes5<double> F;
             for (int q = 0; q < 5; q++) F.p[q] = h->Fd[5 * dir + q];
-            launch_sweep<double, StateD>(h, dir, st, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
         }
         Planes15<double> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Fd[q];
@@ -401,7 +413,7 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
         for (int dir = 0; dir < 3; dir++) {
             Planes5<float> F;
             for (int q = 0; q < 5; q++) F.p[q] = h->Ff[5 * dir + q];
-            launch_sweep<float, StateM>(h, dir, st, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
         }
         Planes15<float> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Ff[q];
