@@ -40,8 +40,9 @@ int check_params(const gkm_grid* g, const gkm_params* p) {
     if (g->nx < 3 || g->ny < 3 || g->nz < 3)
         return fail(GKM_E_ARG, "box %dx%dx%d too small: each axis needs at least 3 interior cells", g->nx, g->ny,
                     g->nz);
+    // 32-bit byte offsets address every plane: at most 2^28 padded cells
     const double padded = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 6);
-    if (padded > 2.0e9) return fail(GKM_E_ARG, "box too large: %.0f padded cells", padded);
+    if (padded > 268435456.0) return fail(GKM_E_ARG, "box too large: %.0f padded cells (limit 2^28)", padded);
     if (!(g->dx > 0) || !std::isfinite(g->dx)) return fail(GKM_E_ARG, "dx must be positive");
     if (!(p->gamma > 1.0)) return fail(GKM_E_ARG, "gamma must exceed 1");
     if (!(p->R > 0) || !(p->Pr > 0) || !(p->mu0 >= 0) || !(p->T0 > 0))
@@ -78,8 +79,8 @@ int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
     g.P0 = grid->ny + 6;
     g.P1 = grid->nx + 6;
     g.P2 = grid->nz + 6;
-    g.s1 = g.P2;
-    g.s0 = (long)g.P1 * g.P2;
+    g.s1 = (unsigned)g.P2;
+    g.s0 = (unsigned)g.P1 * (unsigned)g.P2;
     h->ncell = (long)g.P0 * g.P1 * g.P2;
     h->flux_block = 256;
     hipError_t e = hipGetDevice(&h->device);

@@ -4,15 +4,19 @@
 // mixed form).
 //
 // The flux follows weno/src/GKMflux.cpp:11-215 (the same moments, slope solves
-// and time-integral coefficients) but is reorganised for register pressure:
-// the reference keeps ~400 doubles live (seven [3][7] tables and ~50
-// five-vectors, GPUupdate.cu:499-520).  Here tau, the gamma and p coefficients
-// are formed first, then each Maxwellian (left, right, equilibrium) is visited
-// once and its contributions are accumulated into four 5-vectors (the
-// tangential slope sources of the equilibrium state, the time-slope source and
-// the flux), so a state's tables and slopes die as soon as its block ends.
-// Only the normal axis carries half-space moments (the reference fills them
-// for all three axes, but MCal reads them only from row 0).
+// and time-integral coefficients) but is reorganised:
+//  * each Maxwellian's moments are taken in its tangentially co-moving frame,
+//    where they collapse to a few products of the normal-axis table (see the
+//    block comment above acc_u); the reference's MCal instead expands every
+//    <u^k v^l w^m psi (a.psi)> in the lab frame (~85 flops, 35 calls per
+//    interface);
+//  * tau and the gamma / p time-integral coefficients are formed first, then
+//    each Maxwellian (left, right, equilibrium) is visited once and its
+//    contributions are accumulated into four 5-vectors, so its tables and
+//    slopes die as soon as its block ends (the reference keeps ~400 doubles
+//    live, GPUupdate.cu:499-520);
+//  * only the normal axis carries half-space moments (the reference fills them
+//    for all three axes, but MCal reads them only from row 0).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -131,162 +135,219 @@ template <class T> struct Gas {
     T K;       // internal degrees of freedom
 };
 
-// <u^k v^l w^m psi>, psi = (1, u, v, w, (u^2+v^2+w^2+xi^2)/2)  (GKMflux.cpp:266-275)
-template <int k, int l, int m, class T>
-__device__ __forceinline__ void mom(T Mo[5], const T* n, const T* v, const T* w, T e2) {
-    Mo[0] = n[k] * v[l] * w[m];
-    Mo[1] = n[1 + k] * v[l] * w[m];
-    Mo[2] = n[k] * v[1 + l] * w[m];
-    Mo[3] = n[k] * v[l] * w[1 + m];
-    Mo[4] = T(0.5) * (n[2 + k] * v[l] * w[m] + n[k] * v[2 + l] * w[m] + n[k] * v[l] * w[2 + m] +
-                      n[k] * v[l] * w[m] * e2);
-}
-
-// <u^k v^l w^m psi (a . psi)>  (GKMflux.cpp:225-264), accumulated as acc += s * M
-template <int k, int l, int m, class T>
-__device__ __forceinline__ void mom_a_acc(T acc[5], T s, const T* n, const T* v, const T* w, T e2, T e4,
-                                          const T a[5]) {
-    const T v0 = T(0.5) * (n[2 + k] * v[l] * w[m] + n[k] * v[2 + l] * w[m] + n[k] * v[l] * w[2 + m] +
-                           n[k] * v[l] * w[m] * e2);
-    const T v1 = T(0.5) * (n[3 + k] * v[l] * w[m] + n[1 + k] * v[2 + l] * w[m] + n[1 + k] * v[l] * w[2 + m] +
-                           n[1 + k] * v[l] * w[m] * e2);
-    const T v2 = T(0.5) * (n[2 + k] * v[1 + l] * w[m] + n[k] * v[3 + l] * w[m] + n[k] * v[1 + l] * w[2 + m] +
-                           n[k] * v[1 + l] * w[m] * e2);
-    const T v3 = T(0.5) * (n[2 + k] * v[l] * w[1 + m] + n[k] * v[2 + l] * w[1 + m] + n[k] * v[l] * w[3 + m] +
-                           n[k] * v[l] * w[1 + m] * e2);
-    T m0 = a[0] * (n[k] * v[l] * w[m]) + a[1] * (n[1 + k] * v[l] * w[m]) + a[2] * (n[k] * v[1 + l] * w[m]) +
-           a[3] * (n[k] * v[l] * w[1 + m]) + a[4] * v0;
-    T m1 = a[0] * (n[1 + k] * v[l] * w[m]) + a[1] * (n[2 + k] * v[l] * w[m]) +
-           a[2] * (n[1 + k] * v[1 + l] * w[m]) + a[3] * (n[1 + k] * v[l] * w[1 + m]) + a[4] * v1;
-    T m2 = a[0] * (n[k] * v[1 + l] * w[m]) + a[1] * (n[1 + k] * v[1 + l] * w[m]) +
-           a[2] * (n[k] * v[2 + l] * w[m]) + a[3] * (n[k] * v[1 + l] * w[1 + m]) + a[4] * v2;
-    T m3 = a[0] * (n[k] * v[l] * w[1 + m]) + a[1] * (n[1 + k] * v[l] * w[1 + m]) +
-           a[2] * (n[k] * v[1 + l] * w[1 + m]) + a[3] * (n[k] * v[l] * w[2 + m]) + a[4] * v3;
-    T m4 = T(0.25) * a[4] *
-               (n[4 + k] * v[l] * w[m] + n[k] * v[4 + l] * w[m] + n[k] * v[l] * w[4 + m] + n[k] * v[l] * w[m] * e4 +
-                2 * n[2 + k] * v[2 + l] * w[m] + 2 * n[2 + k] * v[l] * w[2 + m] + 2 * n[2 + k] * v[l] * w[m] * e2 +
-                2 * n[k] * v[2 + l] * w[2 + m] + 2 * n[k] * v[2 + l] * w[m] * e2 + 2 * n[k] * v[l] * w[2 + m] * e2) +
-           a[0] * v0 + a[1] * v1 + a[2] * v2 + a[3] * v3;
-    acc[0] += s * m0;
-    acc[1] += s * m1;
-    acc[2] += s * m2;
-    acc[3] += s * m3;
-    acc[4] += s * m4;
-}
-
-// Closed-form inverse of the Maxwellian moment matrix: b -> a (GKMflux.cpp:280-296);
-// h = 1/(2 lam).
-template <class T>
-__device__ __forceinline__ void slope_solve(const Gas<T>& g, const T b[5], const T U[3], T lam, T h, T a[5]) {
-    const T uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
-    const T R2 = b[1] - U[0] * b[0];
-    const T R3 = b[2] - U[1] * b[0];
-    const T R4 = b[3] - U[2] * b[0];
-    const T q = g.kp3 * h;  // (K+3)/(2 lam)
-    const T R5 = 2 * b[4] - b[0] * (uu + q);
-    a[4] = g.invPr * (R5 - 2 * U[0] * R2 - 2 * U[1] * R3 - 2 * U[2] * R4) * (4 * lam * lam) * g.ikp3;
-    a[3] = 2 * lam * R4 - U[2] * a[4];
-    a[2] = 2 * lam * R3 - U[1] * a[4];
-    a[1] = 2 * lam * R2 - U[0] * a[4];
-    a[0] = b[0] - a[1] * U[0] - a[2] * U[1] - a[3] * U[2] - T(0.5) * a[4] * (uu + q);
-}
-
-// One Maxwellian: primitive variables and the full-space tables along the
-// three axes (index 0..6).
+// ---------------------------------------------------------------------------
+// Moments in the tangentially co-moving frame.
+//
+// For a Maxwellian with velocity (U, V, W) and lambda, h = 1/(2 lambda), use
+// psi' = (1, u, v', w', (u^2 + v'^2 + w'^2 + xi^2)/2) with v' = v - V,
+// w' = w - W.  The tangential moments are then central Gaussian moments
+// (<v'> = <v'^3> = 0, <v'^2> = h, <v'^4> = 3h^2) and xi^2 ~ h chi^2_K, so
+// every moment the flux needs, <u^k v'^l w'^m psi' (a' . psi')> with l, m in
+// {0, 1}, reduces to a few products of the normal-axis table N_j = <u^j>
+// (full or half space) and the sums
+//   Q_k  = <u^k S>    = (N_{k+2} + (K+2) h N_k) / 2,     S = psi'_4
+//   R_k  = <u^k S^2>  = (N_{k+4} + 2 (K+2) h N_{k+2} + (K+2)(K+4) h^2 N_k) / 4
+//   Q'_k = <u^k v'^2 S> / h = (N_{k+2} + (K+4) h N_k) / 2.
+// Vectors move between psi and psi' by exact linear maps: moments / sources
+// with Tinv (to the frame) and T (back to the lab frame), slopes with T^T.
+// The slope solve of GKMflux.cpp:280-296 (1/Pr on a[4] included) commutes
+// with that change of frame, so slopes are solved directly in the frame with
+// velocity (U, 0, 0).  This is the same flux as GKMflux.cpp:11-215; the
+// reference's MCal evaluates the lab-frame products term by term instead.
+// ---------------------------------------------------------------------------
 template <class T> struct Maxw {
-    T den, U[3], lam, h;  // h = 1/(2 lam)
-    T e2, e4;
+    T den, U, V, W, lam, h;
+};
+
+template <class T> struct NTab {
+    T N[7];
+    T Q[4], R[3], Qp[2];
 };
 
 template <class T>
 __device__ __forceinline__ void maxw_from_W(const Gas<T>& g, const T W[5], Maxw<T>& s) {
     s.den = W[0];
     const T ri = M<T>::rcp(W[0]);
-    s.U[0] = W[1] * ri;
-    s.U[1] = W[2] * ri;
-    s.U[2] = W[3] * ri;
-    const T P = (s.den * (g.gam - 1)) * (W[4] * ri - T(0.5) * (s.U[0] * s.U[0] + s.U[1] * s.U[1] + s.U[2] * s.U[2]));
+    s.U = W[1] * ri;
+    s.V = W[2] * ri;
+    s.W = W[3] * ri;
+    const T P = (s.den * (g.gam - 1)) * (W[4] * ri - T(0.5) * (s.U * s.U + s.V * s.V + s.W * s.W));
     s.lam = T(0.5) * s.den * M<T>::rcp(P);
     s.h = M<T>::rcp(2 * s.lam);
-    s.e2 = g.K * s.h;                                  // K/(2 lam)
-    s.e4 = (3 * g.K + g.K * (g.K - 1)) * s.h * s.h;    // (3K + K(K-1))/(4 lam^2)
 }
 
-template <class T> __device__ __forceinline__ void full_table(T U, T h, T f[7]) {
-    f[0] = T(1);
-    f[1] = U;
+template <class T> __device__ __forceinline__ void ntab_sums(const Gas<T>& g, T h, NTab<T>& t) {
+    const T c2 = (g.K + 2) * h, c4 = (g.K + 4) * h, c24 = (g.K + 2) * (g.K + 4) * h * h;
 #pragma unroll
-    for (int i = 2; i < 7; i++) f[i] = U * f[i - 1] + f[i - 2] * T(i - 1) * h;
+    for (int k = 0; k < 4; k++) t.Q[k] = T(0.5) * (t.N[k + 2] + c2 * t.N[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) t.R[k] = T(0.25) * (t.N[k + 4] + 2 * c2 * t.N[k + 2] + c24 * t.N[k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) t.Qp[k] = T(0.5) * (t.N[k + 2] + c4 * t.N[k]);
 }
 
-// half-space moments along the normal axis; sgn = +1 for u > 0, -1 for u < 0
-template <class T> __device__ __forceinline__ void half_table(T U, T lam, T h, int sgn, T f[7]) {
+// full-space normal-axis table <u^j>
+template <class T> __device__ __forceinline__ void full_ntab(const Gas<T>& g, T U, T h, NTab<T>& t) {
+    t.N[0] = T(1);
+    t.N[1] = U;
+#pragma unroll
+    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + t.N[i - 2] * T(i - 1) * h;
+    ntab_sums(g, h, t);
+}
+
+// half-space normal-axis table; sgn = +1 for u > 0, -1 for u < 0 (GKMflux.cpp:54-65)
+template <class T>
+__device__ __forceinline__ void half_ntab(const Gas<T>& g, T U, T lam, T h, int sgn, NTab<T>& t) {
     const T sl = M<T>::sqrt_(lam);
     const T pi = T(3.141592653589793238462643383279502884);
-    f[0] = T(0.5) * M<T>::erfc_(-T(sgn) * sl * U);
-    f[1] = U * f[0] + T(sgn) * T(0.5) * M<T>::exp_(-lam * U * U) * M<T>::rcp(M<T>::sqrt_(lam * pi));
+    t.N[0] = T(0.5) * M<T>::erfc_(-T(sgn) * sl * U);
+    t.N[1] = U * t.N[0] + T(sgn) * T(0.5) * M<T>::exp_(-lam * U * U) * M<T>::rcp(M<T>::sqrt_(lam * pi));
 #pragma unroll
-    for (int i = 2; i < 7; i++) f[i] = U * f[i - 1] + f[i - 2] * T(i - 1) * h;
+    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + t.N[i - 2] * T(i - 1) * h;
+    ntab_sums(g, h, t);
 }
 
-// Contribution of one side (left = sgn +1, right = sgn -1) to the flux.
-//   W     : reconstructed interface state of that side
-//   DWx/y/z: slopes of that side (normal, tangential-1, tangential-2)
-// Accumulates: by, bz (sources of the equilibrium tangential slopes),
-//   B (source of the equilibrium time slope), F (the flux).
+// acc += s <u^k psi' (a . psi')>
+template <int k, class T>
+__device__ __forceinline__ void acc_u(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
+    acc[0] += s * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Q[k]);
+    acc[1] += s * (a[0] * t.N[k + 1] + a[1] * t.N[k + 2] + a[4] * t.Q[k + 1]);
+    const T sh = s * h * t.N[k];
+    acc[2] += sh * a[2];
+    acc[3] += sh * a[3];
+    acc[4] += s * (a[0] * t.Q[k] + a[1] * t.Q[k + 1] + a[4] * t.R[k]);
+}
+
+// acc += s <u^k v' psi' (a . psi')>
+template <int k, class T>
+__device__ __forceinline__ void acc_v(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
+    const T sh = s * h;
+    acc[0] += sh * a[2] * t.N[k];
+    acc[1] += sh * a[2] * t.N[k + 1];
+    acc[2] += sh * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Qp[k]);
+    acc[4] += sh * a[2] * t.Qp[k];
+}
+
+// acc += s <u^k w' psi' (a . psi')>
+template <int k, class T>
+__device__ __forceinline__ void acc_w(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
+    const T sh = s * h;
+    acc[0] += sh * a[3] * t.N[k];
+    acc[1] += sh * a[3] * t.N[k + 1];
+    acc[3] += sh * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Qp[k]);
+    acc[4] += sh * a[3] * t.Qp[k];
+}
+
+// moment / source vector: lab psi basis -> co-moving frame (Tinv)
+template <class T> __device__ __forceinline__ void to_frame(const T b[5], T V, T W, T o[5]) {
+    o[0] = b[0];
+    o[1] = b[1];
+    o[2] = b[2] - V * b[0];
+    o[3] = b[3] - W * b[0];
+    o[4] = b[4] - V * b[2] - W * b[3] + T(0.5) * (V * V + W * W) * b[0];
+}
+
+// moment vector: co-moving frame -> lab psi basis (T), added to acc
+template <class T> __device__ __forceinline__ void add_to_lab(T acc[5], const T m[5], T V, T W) {
+    acc[0] += m[0];
+    acc[1] += m[1];
+    acc[2] += m[2] + V * m[0];
+    acc[3] += m[3] + W * m[0];
+    acc[4] += m[4] + V * m[2] + W * m[3] + T(0.5) * (V * V + W * W) * m[0];
+}
+
+// slope solve in the co-moving frame (velocity (U, 0, 0)); b, a in psi'
 template <class T>
-__device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, const T* hs, const T DWx[5],
+__device__ __forceinline__ void solve_frame(const Gas<T>& g, const T b[5], T U, T lam, T h, T a[5]) {
+    const T q = U * U + g.kp3 * h;  // U^2 + (K+3)/(2 lam)
+    const T R2 = b[1] - U * b[0];
+    a[4] = g.invPr * (2 * b[4] - b[0] * q - 2 * U * R2) * (4 * lam * lam) * g.ikp3;
+    a[3] = 2 * lam * b[3];
+    a[2] = 2 * lam * b[2];
+    a[1] = 2 * lam * R2 - U * a[4];
+    a[0] = b[0] - a[1] * U - T(0.5) * a[4] * q;
+}
+
+// Contribution of one side (left: normal table u > 0, right: u < 0).
+// Accumulates, in the lab frame: by, bz (sources of the equilibrium state's
+// tangential slopes), B (source of its time slope) and F (the flux).
+template <class T>
+__device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, const NTab<T>& nh, const T DWx[5],
                                            const T DWy[5], const T DWz[5], T c45, T c5, T p3, T p4, T p5, T by[5],
                                            T bz[5], T B[5], T F[5]) {
-    T fn[7], fv[7], fw[7];
-    full_table(s.U[0], s.h, fn);
-    full_table(s.U[1], s.h, fv);
-    full_table(s.U[2], s.h, fw);
+    const T h = s.h;
     const T ri = M<T>::rcp(s.den);
-    T b[5], ax[5], ay[5], az[5], A[5];
+    T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWx[q] * ri;
-    slope_solve(g, b, s.U, s.lam, s.h, ax);
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, ax);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWy[q] * ri;
-    slope_solve(g, b, s.U, s.lam, s.h, ay);
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, ay);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWz[q] * ri;
-    slope_solve(g, b, s.U, s.lam, s.h, az);
-    // time slope of this side: A = -(<u psi ax> + <v psi ay> + <w psi az>) over the full space
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, az);
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = T(0);
-    mom_a_acc<1, 0, 0>(b, T(-1), fn, fv, fw, s.e2, s.e4, ax);
-    mom_a_acc<0, 1, 0>(b, T(-1), fn, fv, fw, s.e2, s.e4, ay);
-    mom_a_acc<0, 0, 1>(b, T(-1), fn, fv, fw, s.e2, s.e4, az);
-    slope_solve(g, b, s.U, s.lam, s.h, A);
-    // equilibrium tangential-slope sources: <psi ay>_half, <psi az>_half
-    mom_a_acc<0, 0, 0>(by, T(1), hs, fv, fw, s.e2, s.e4, ay);
-    mom_a_acc<0, 0, 0>(bz, T(1), hs, fv, fw, s.e2, s.e4, az);
-    const T d = s.den;
-    // time-slope source: (gm4+gm5) rho (<u psi ax> + <v psi ay> + <w psi az>)_half + gm5 rho <psi A>_half
-    const T dc45 = d * c45;
-    mom_a_acc<1, 0, 0>(B, dc45, hs, fv, fw, s.e2, s.e4, ax);
-    mom_a_acc<0, 1, 0>(B, dc45, hs, fv, fw, s.e2, s.e4, ay);
-    mom_a_acc<0, 0, 1>(B, dc45, hs, fv, fw, s.e2, s.e4, az);
-    mom_a_acc<0, 0, 0>(B, d * c5, hs, fv, fw, s.e2, s.e4, A);
-    // flux: p3 rho <u psi> + p4 rho (<u^2 psi ax> + <uv psi ay> + <uw psi az>) + p5 rho <u psi A>
+    for (int q = 0; q < 5; q++) sv[q] = s.V * ay[q] + s.W * az[q];
+    // time slope: A = -(<u psi ax> + <v psi ay> + <w psi az>) over the full space
+    {
+        NTab<T> nf;
+        full_ntab(g, s.U, h, nf);
+#pragma unroll
+        for (int q = 0; q < 5; q++) b[q] = T(0);
+        acc_u<1>(b, T(-1), nf, h, ax);
+        acc_v<0>(b, T(-1), nf, h, ay);
+        acc_w<0>(b, T(-1), nf, h, az);
+        acc_u<0>(b, T(-1), nf, h, sv);
+        solve_frame(g, b, s.U, s.lam, h, A);
+    }
     T m[5];
-    mom<1, 0, 0>(m, hs, fv, fw, s.e2);
-    const T dp3 = d * p3;
+    // equilibrium tangential-slope sources <psi ay>_half, <psi az>_half
 #pragma unroll
-    for (int q = 0; q < 5; q++) F[q] += dp3 * m[q];
+    for (int q = 0; q < 5; q++) m[q] = T(0);
+    acc_u<0>(m, T(1), nh, h, ay);
+    add_to_lab(by, m, s.V, s.W);
+#pragma unroll
+    for (int q = 0; q < 5; q++) m[q] = T(0);
+    acc_u<0>(m, T(1), nh, h, az);
+    add_to_lab(bz, m, s.V, s.W);
+    const T d = s.den;
+    // time-slope source: (gm4+gm5) rho (<u psi ax> + <v psi ay> + <w psi az>) + gm5 rho <psi A>
+    T c[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        m[q] = T(0);
+        c[q] = c45 * sv[q] + c5 * A[q];
+    }
+    const T dc45 = d * c45;
+    acc_u<1>(m, dc45, nh, h, ax);
+    acc_v<0>(m, dc45, nh, h, ay);
+    acc_w<0>(m, dc45, nh, h, az);
+    acc_u<0>(m, d, nh, h, c);
+    add_to_lab(B, m, s.V, s.W);
+    // flux: p3 rho <u psi> + p4 rho (<u^2 psi ax> + <uv psi ay> + <uw psi az>) + p5 rho <u psi A>
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        m[q] = T(0);
+        c[q] = p4 * sv[q] + p5 * A[q];
+    }
+    c[0] += p3;
     const T dp4 = d * p4;
-    mom_a_acc<2, 0, 0>(F, dp4, hs, fv, fw, s.e2, s.e4, ax);
-    mom_a_acc<1, 1, 0>(F, dp4, hs, fv, fw, s.e2, s.e4, ay);
-    mom_a_acc<1, 0, 1>(F, dp4, hs, fv, fw, s.e2, s.e4, az);
-    mom_a_acc<1, 0, 0>(F, d * p5, hs, fv, fw, s.e2, s.e4, A);
+    acc_u<2>(m, dp4, nh, h, ax);
+    acc_v<1>(m, dp4, nh, h, ay);
+    acc_w<1>(m, dp4, nh, h, az);
+    acc_u<1>(m, d, nh, h, c);
+    add_to_lab(F, m, s.V, s.W);
 }
 
 // BGK-NS interface flux, time-integrated over dt (GKMflux.cpp:11-215).
 // All vectors are in the rotated slot order (slot 1 = normal momentum).
-// tau0 = mu/p of the left cell.  DW(l|r) = slopes left / right of the
-// interface along the normal (x) and the two tangential (y, z) directions.
+// tau0 = mu/p of the left cell.  slopes(side, Dx, Dy, Dz) yields the slopes
+// left (side 0) or right (side 1) of the interface along the normal (x) and
+// the two tangential (y, z) directions.
 template <class T, class LoadSlopes>
 __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T WR[5], const T Wl[5],
                                          const T Wr[5], LoadSlopes&& slopes, T tau0, T dt, T dx, T F[5]) {
@@ -314,23 +375,20 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     const T p4 = -Q - tau * P;
     const T p5 = -tau * P;
 
-    // half-space normal tables of the two sides and the equilibrium state
-    T hl[7], hr[7];
-    half_table(l.U[0], l.lam, l.h, +1, hl);
-    half_table(r.U[0], r.lam, r.h, -1, hr);
-    T We[5];
+    // half-space normal tables of the two sides and the equilibrium state We
+    NTab<T> hl, hr;
+    half_ntab(g, l.U, l.lam, l.h, +1, hl);
+    half_ntab(g, r.U, r.lam, r.h, -1, hr);
+    T We[5] = {T(0), T(0), T(0), T(0), T(0)};
     {
-        T fv[7], fw[7], m[5];
-        full_table(l.U[1], l.h, fv);
-        full_table(l.U[2], l.h, fw);
-        mom<0, 0, 0>(m, hl, fv, fw, l.e2);
+        const T e0[5] = {T(1), T(0), T(0), T(0), T(0)};
+        T m[5] = {T(0), T(0), T(0), T(0), T(0)};
+        acc_u<0>(m, l.den, hl, l.h, e0);
+        add_to_lab(We, m, l.V, l.W);
 #pragma unroll
-        for (int q = 0; q < 5; q++) We[q] = l.den * m[q];
-        full_table(r.U[1], r.h, fv);
-        full_table(r.U[2], r.h, fw);
-        mom<0, 0, 0>(m, hr, fv, fw, r.e2);
-#pragma unroll
-        for (int q = 0; q < 5; q++) We[q] += r.den * m[q];
+        for (int q = 0; q < 5; q++) m[q] = T(0);
+        acc_u<0>(m, r.den, hr, r.h, e0);
+        add_to_lab(We, m, r.V, r.W);
     }
     Maxw<T> e;
     maxw_from_W(g, We, e);
@@ -355,53 +413,60 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
         side_block(g, r, hr, Dx, Dy, Dz, c45, gm5, p3, p4, p5, by, bz, B, Fa);
     }
 
-    // equilibrium state: normal slopes from the cell averages, tangential slopes from by, bz
-    T he_p[7], he_n[7], fn[7], fv[7], fw[7];
-    half_table(e.U[0], e.lam, e.h, +1, he_p);
-    half_table(e.U[0], e.lam, e.h, -1, he_n);
-    full_table(e.U[0], e.h, fn);
-    full_table(e.U[1], e.h, fv);
-    full_table(e.U[2], e.h, fw);
-    T axl[5], axr[5], ay[5], az[5], A[5], b[5];
+    // equilibrium state, in its own co-moving frame
+    const T he = e.h;
+    T axl[5], axr[5], ay[5], az[5], A[5], b[5], bp[5], sv[5];
     const T s2 = T(2) * M<T>::rcp(e.den * dx);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * s2;
-    slope_solve(g, b, e.U, e.lam, e.h, axl);
+    to_frame(b, e.V, e.W, bp);
+    solve_frame(g, bp, e.U, e.lam, he, axl);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * s2;
-    slope_solve(g, b, e.U, e.lam, e.h, axr);
-    slope_solve(g, by, e.U, e.lam, e.h, ay);
-    slope_solve(g, bz, e.U, e.lam, e.h, az);
-    const T de = e.den;
-    {
-        T m[5];
-        mom<0, 0, 0>(m, fn, fv, fw, e.e2);
-        const T dg1 = de * gm1;
+    to_frame(b, e.V, e.W, bp);
+    solve_frame(g, bp, e.U, e.lam, he, axr);
+    to_frame(by, e.V, e.W, bp);
+    solve_frame(g, bp, e.U, e.lam, he, ay);
+    to_frame(bz, e.V, e.W, bp);
+    solve_frame(g, bp, e.U, e.lam, he, az);
 #pragma unroll
-        for (int q = 0; q < 5; q++) B[q] += dg1 * m[q];
+    for (int q = 0; q < 5; q++) sv[q] = e.V * ay[q] + e.W * az[q];
+    const T de = e.den;
+    NTab<T> ep, en, ef;
+    half_ntab(g, e.U, e.lam, he, +1, ep);
+    half_ntab(g, e.U, e.lam, he, -1, en);
+    full_ntab(g, e.U, he, ef);
+    // time-slope source of the equilibrium part, added in the frame
+    to_frame(B, e.V, e.W, bp);
+    {
+        const T dg2 = de * gm2;
+        T c[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) c[q] = gm2 * sv[q];
+        c[0] += gm1;
+        acc_u<0>(bp, de, ef, he, c);
+        acc_u<1>(bp, dg2, ep, he, axl);
+        acc_u<1>(bp, dg2, en, he, axr);
+        acc_v<0>(bp, dg2, ef, he, ay);
+        acc_w<0>(bp, dg2, ef, he, az);
     }
-    const T dg2 = de * gm2;
-    mom_a_acc<1, 0, 0>(B, dg2, he_p, fv, fw, e.e2, e.e4, axl);
-    mom_a_acc<1, 0, 0>(B, dg2, he_n, fv, fw, e.e2, e.e4, axr);
-    mom_a_acc<0, 1, 0>(B, dg2, fn, fv, fw, e.e2, e.e4, ay);
-    mom_a_acc<0, 0, 1>(B, dg2, fn, fv, fw, e.e2, e.e4, az);
     const T ide = M<T>::rcp(de);
 #pragma unroll
-    for (int q = 0; q < 5; q++) B[q] *= ide;
-    slope_solve(g, B, e.U, e.lam, e.h, A);
+    for (int q = 0; q < 5; q++) bp[q] *= ide;
+    solve_frame(g, bp, e.U, e.lam, he, A);
     {
-        T m[5];
-        mom<1, 0, 0>(m, fn, fv, fw, e.e2);
-        const T dp0 = de * p0;
+        T m[5] = {T(0), T(0), T(0), T(0), T(0)}, c[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) Fa[q] += dp0 * m[q];
+        for (int q = 0; q < 5; q++) c[q] = p1 * A[q] + p2 * sv[q];
+        c[0] += p0;
+        const T dp2 = de * p2;
+        acc_u<1>(m, de, ef, he, c);
+        acc_u<2>(m, dp2, ep, he, axl);
+        acc_u<2>(m, dp2, en, he, axr);
+        acc_v<1>(m, dp2, ef, he, ay);
+        acc_w<1>(m, dp2, ef, he, az);
+        add_to_lab(Fa, m, e.V, e.W);
     }
-    mom_a_acc<1, 0, 0>(Fa, de * p1, fn, fv, fw, e.e2, e.e4, A);
-    const T dp2 = de * p2;
-    mom_a_acc<2, 0, 0>(Fa, dp2, he_p, fv, fw, e.e2, e.e4, axl);
-    mom_a_acc<2, 0, 0>(Fa, dp2, he_n, fv, fw, e.e2, e.e4, axr);
-    mom_a_acc<1, 1, 0>(Fa, dp2, fn, fv, fw, e.e2, e.e4, ay);
-    mom_a_acc<1, 0, 1>(Fa, dp2, fn, fv, fw, e.e2, e.e4, az);
 #pragma unroll
     for (int q = 0; q < 5; q++) F[q] = Fa[q];
 }

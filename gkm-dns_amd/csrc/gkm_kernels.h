@@ -13,40 +13,84 @@
 
 namespace gkm {
 
-// Padded geometry: extents P0 (y), P1 (x), P2 (z) and strides.
+// Padded geometry: extents P0 (y), P1 (x), P2 (z) and strides.  Cell indices
+// are 32-bit (the handle limits a plane to 2^28 cells, so byte offsets of FP64
+// planes fit 31 bits): a load is then a 32-bit offset from a plane base held
+// in scalar registers, one offset register serving every plane.
 struct Geo {
     int P0, P1, P2;
-    long s0, s1;  // s2 = 1
+    unsigned s0, s1;  // s2 = 1
     int nx, ny, nz;
 };
+
+// Division by a run-time invariant divisor d >= 1 for dividends t < 2^31:
+// q = (umulhi(t, m) + t) >> s with s = ceil(log2 d),
+// m = floor(2^32 (2^s - d) / d) + 1 (round-up reciprocal, 33-bit magic).
+struct FastDiv {
+    unsigned d, m, s;
+};
+
+inline FastDiv make_fastdiv(unsigned d) {
+    FastDiv f;
+    f.d = d;
+    f.s = 0;
+    while ((1ull << f.s) < d) f.s++;
+    f.m = (unsigned)((((unsigned long long)1 << 32) * (((unsigned long long)1 << f.s) - d)) / d + 1);
+    return f;
+}
+
+__device__ __forceinline__ unsigned fdiv(unsigned t, const FastDiv& f) {
+    const unsigned hi = __umulhi(t, f.m);
+    return (hi + t) >> f.s;
+}
 
 // A box of cells/interfaces [o, o+n) in padded coordinates, flattened with the
 // unit-stride axis fastest.
 struct Box3 {
     int o0, o1, o2;
     int n0, n1, n2;
-    __host__ __device__ long count() const { return (long)n0 * n1 * n2; }
+    FastDiv f1, f2;  // division by n1 and n2
+    __host__ __device__ unsigned count() const { return (unsigned)n0 * n1 * n2; }
 };
 
-__device__ __forceinline__ long box_index(const Geo& g, const Box3& b, long t) {
-    int k = (int)(t % b.n2);
-    long r = t / b.n2;
-    int i = (int)(r % b.n1);
-    int j = (int)(r / b.n1);
-    return (long)(j + b.o0) * g.s0 + (long)(i + b.o1) * g.s1 + (k + b.o2);
+inline Box3 make_box(int o0, int o1, int o2, int n0, int n1, int n2) {
+    Box3 b;
+    b.o0 = o0; b.o1 = o1; b.o2 = o2;
+    b.n0 = n0; b.n1 = n1; b.n2 = n2;
+    b.f1 = make_fastdiv((unsigned)n1);
+    b.f2 = make_fastdiv((unsigned)n2);
+    return b;
+}
+
+__device__ __forceinline__ unsigned box_index(const Geo& g, const Box3& b, unsigned t) {
+    const unsigned r = fdiv(t, b.f2);
+    const unsigned k = t - r * (unsigned)b.n2;
+    const unsigned j = fdiv(r, b.f1);
+    const unsigned i = r - j * (unsigned)b.n1;
+    return (j + b.o0) * g.s0 + (i + b.o1) * g.s1 + (k + b.o2);
+}
+
+// element c of a plane, addressed by a 32-bit byte offset
+template <class T> __device__ __forceinline__ T ld(const T* base, unsigned c) {
+    return *(const T*)((const char*)base + c * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ void put(T* base, unsigned c, T v) {
+    *(T*)((char*)base + c * (unsigned)sizeof(T)) = v;
 }
 
 // ---- state storage ---------------------------------------------------------
 struct StateD {
     double* w[5];
-    __device__ __forceinline__ double get(int q, long c) const { return w[q][c]; }
-    __device__ __forceinline__ double cell(int q, long c) const { return w[q][c]; }
+    __device__ __forceinline__ double get(int q, unsigned c) const { return ld(w[q], c); }
+    __device__ __forceinline__ double cell(int q, unsigned c) const { return ld(w[q], c); }
 };
 struct StateM {
     float* hi[5];
     float* lo[5];
-    __device__ __forceinline__ double get(int q, long c) const { return (double)hi[q][c] + (double)lo[q][c]; }
-    __device__ __forceinline__ float cell(int q, long c) const { return hi[q][c]; }
+    __device__ __forceinline__ double get(int q, unsigned c) const {
+        return (double)ld(hi[q], c) + (double)ld(lo[q], c);
+    }
+    __device__ __forceinline__ float cell(int q, unsigned c) const { return ld(hi[q], c); }
 };
 
 template <class T> struct Planes5 {

@@ -1,6 +1,7 @@
 // gkm_step.hip — kernels of one GKM-DNS time step on gfx950 and their
-// launchers: periodic ghost fill, WENO reconstruction per sweep, BGK flux per
-// sweep, conservative update, state split/merge for the mixed form.
+// launchers: periodic ghost fill fused with the per-cell primitives, WENO
+// reconstruction per sweep, BGK flux per sweep, conservative update, state
+// split/merge for the mixed form.
 #include <hip/hip_runtime.h>
 
 #include "gkm_internal.h"
@@ -26,32 +27,32 @@ __device__ __forceinline__ double viscosity(const StepPar& par, double T) {
 }
 
 template <int NPL, class T, class Real, class St>
-__global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Planes10<T> f, St st, Real* Tw, Real* tau, StepPar par) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = (long)g.P0 * g.P1 * g.P2;
-    if (t >= n) return;
-    int k = (int)(t % g.P2);
-    long r = t / g.P2;
-    int i = (int)(r % g.P1);
-    int j = (int)(r / g.P1);
+__global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Box3 all, Planes10<T> f, St st, Real* Tw, Real* tau,
+                                                    StepPar par) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= all.count()) return;
+    const unsigned r = fdiv(t, all.f2);
+    const int k = (int)(t - r * (unsigned)all.n2);
+    const int j = (int)fdiv(r, all.f1);
+    const int i = (int)(r - (unsigned)j * (unsigned)all.n1);
     const bool gj = j < 3 || j >= g.ny + 3, gi = i < 3 || i >= g.nx + 3, gk = k < 3 || k >= g.nz + 3;
-    long src = t;
+    unsigned src = t;
     if (gj || gi || gk) {
-        int js = j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
-        int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
-        int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
-        src = (long)js * g.s0 + (long)is * g.s1 + ks;
+        const int js = j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
+        const int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
+        const int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
+        src = (unsigned)js * g.s0 + (unsigned)is * g.s1 + (unsigned)ks;
 #pragma unroll
-        for (int q = 0; q < NPL; q++) f.p[q][t] = f.p[q][src];
+        for (int q = 0; q < NPL; q++) put(f.p[q], t, ld(f.p[q], src));
     }
     if (!Tw) return;
     const double r0 = st.get(0, src), m1 = st.get(1, src), m2 = st.get(2, src), m3 = st.get(3, src),
                  E = st.get(4, src);
-    Tw[t] = (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R);
+    put(Tw, t, (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R));
     const double u = m1 / r0, v = m2 / r0, w = m3 / r0;
     const double P = (r0 * (par.gam - 1)) * (E / r0 - 0.5 * (u * u + v * v + w * w));
     const double Ti = P / (par.R * r0);
-    tau[t] = (Real)(viscosity(par, Ti) / P);
+    put(tau, t, (Real)(viscosity(par, Ti) / P));
 }
 
 // ---------------------------------------------------------------------------
@@ -60,37 +61,99 @@ __global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Planes10<T> f, St st,
 // evaluates the line of every transverse neighbour again inside each
 // interface (5 lines per interface, update.cpp:247-251); the flux kernel
 // reads the neighbours' values from here instead, which are the same numbers.
+//
+// Temperature mode (update.cpp:560-621): rho, rho u, rho v, rho w by WENO, the
+// temperature (from the per-cell plane) by WENO, rho E rebuilt from both.
 // ---------------------------------------------------------------------------
-template <class Real, class St, int RECON>
-__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, long sn, St st, const Real* __restrict__ Tw,
-                                               Planes10<Real> wi, StepPar par) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const long c = box_index(g, b, t);
-    const Real eps = Real(1e-10);
-    Real wl[5], wr[5];
-    const int nlin = par.energy_recon == 1 ? 5 : 4;
-    for (int q = 0; q < nlin; q++) {
-        Real v[6];
+template <int RECON, int ENERGY, class Real>
+__device__ __forceinline__ void recon_one(const Real (&v)[6][6], Real eps, const StepPar& par, Real wl[5],
+                                          Real wr[5]) {
+    constexpr int nlin = ENERGY == 1 ? 5 : 4;
 #pragma unroll
-        for (int s = 0; s < 6; s++) v[s] = (Real)st.cell(q, c + (s - 2) * sn);
-        weno_line<RECON>(v, eps, wl[q], wr[q]);
-    }
-    if (par.energy_recon == 0) {
-        // temperature reconstruction (update.cpp:560-621), cell temperatures from the plane
-        Real Tc[6];
-#pragma unroll
-        for (int s = 0; s < 6; s++) Tc[s] = Tw[c + (s - 2) * sn];
+    for (int q = 0; q < nlin; q++) weno_line<RECON>(v[q], eps, wl[q], wr[q]);
+    if (ENERGY == 0) {
         Real Tl, Tr;
-        weno_line<RECON>(Tc, eps, Tl, Tr);
+        weno_line<RECON>(v[5], eps, Tl, Tr);
         const Real cR = Real(par.R / (par.gam - 1.0));
         wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
         wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
     }
+}
+
+// Unit-stride sweep (Z): one interface per thread; neighbouring threads share
+// their six-cell stencils through the cache.
+template <class Real, class St, int RECON, int ENERGY>
+__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st, const Real* __restrict__ Tw,
+                                               Planes10<Real> wi, StepPar par) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const unsigned c = box_index(g, b, t);
+    Real v[6][6];
+    constexpr int nlin = ENERGY == 1 ? 5 : 4;
+#pragma unroll
+    for (int q = 0; q < nlin; q++)
+#pragma unroll
+        for (int s = 0; s < 6; s++) v[q][s] = (Real)st.cell(q, c + s * sn - 2 * sn);
+    if (ENERGY == 0)
+#pragma unroll
+        for (int s = 0; s < 6; s++) v[5][s] = ld(Tw, c + s * sn - 2 * sn);
+    Real wl[5], wr[5];
+    recon_one<RECON, ENERGY>(v, Real(1e-10), par, wl, wr);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        wi.p[q][c] = wl[q];
-        wi.p[5 + q][c] = wr[q];
+        put(wi.p[q], c, wl[q]);
+        put(wi.p[5 + q], c, wr[q]);
+    }
+}
+
+// Strided sweeps (X, Y): a thread walks CHUNK consecutive interfaces along the
+// sweep normal with a six-cell register window, so each cell is loaded once
+// per chunk instead of six times; threads run along z (unit stride) for
+// coalescing.  Box axes: n0 = chunks, n1 = the other tangential axis, n2 = z.
+template <class Real, class St, int RECON, int ENERGY, int CHUNK>
+__global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, unsigned sa, int n_normal, St st,
+                                                   const Real* __restrict__ Tw, Planes10<Real> wi, StepPar par) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const unsigned r = fdiv(t, b.f2);
+    const unsigned k = t - r * (unsigned)b.n2;
+    const unsigned chunk = fdiv(r, b.f1);
+    const unsigned a = r - chunk * (unsigned)b.n1;
+    const int first = (int)chunk * CHUNK;                 // normal index offset from 2
+    const int count = min(CHUNK, n_normal - first);
+    unsigned c = (unsigned)(first + 2) * sn + (a + 2) * sa + (k + 2);
+    constexpr bool tmode = ENERGY == 0;
+    constexpr int nlin = tmode ? 4 : 5;
+    Real v[6][6];
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+#pragma unroll
+        for (int s = 0; s < 6; s++) v[q][s] = Real(0);
+#pragma unroll
+    for (int q = 0; q < nlin; q++)
+#pragma unroll
+        for (int s = 0; s < 5; s++) v[q][s + 1] = (Real)st.cell(q, c + s * sn - 2 * sn);
+    if (tmode)
+#pragma unroll
+        for (int s = 0; s < 5; s++) v[5][s + 1] = ld(Tw, c + s * sn - 2 * sn);
+#pragma unroll 1
+    for (int it = 0; it < count; it++) {
+        // slide: drop the oldest cell, load the one three cells ahead
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+#pragma unroll
+            for (int s = 0; s < 5; s++) v[q][s] = v[q][s + 1];
+#pragma unroll
+        for (int q = 0; q < nlin; q++) v[q][5] = (Real)st.cell(q, c + 3 * sn);
+        if (tmode) v[5][5] = ld(Tw, c + 3 * sn);
+        Real wl[5], wr[5];
+        recon_one<RECON, ENERGY>(v, Real(1e-10), par, wl, wr);
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            put(wi.p[q], c, wl[q]);
+            put(wi.p[5 + q], c, wr[q]);
+        }
+        c += sn;
     }
 }
 
@@ -106,14 +169,17 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 // tangential [3, P-4]); writes the five flux planes of that direction.
 // st1 / st2 are the strides of the tangential axes paired with the y / z
 // slots of the rotated frame.
+// FP64: at least 2 waves per SIMD (256 registers); the unconstrained build
+// needs 260 and runs at 1 wave, about 1.5x slower.  FP32 fits 3 waves without
+// a bound; forcing 4 spills.
 // ---------------------------------------------------------------------------
 template <class Real, class St, int DIR>
-__global__ void __launch_bounds__(256) k_flux(Geo g, Box3 b, long sn, long st1, long st2, St st,
-                                              const Real* __restrict__ tau, Planes10<Real> wi, Planes5<Real> F,
-                                              StepPar par) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
+    k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
+           Planes10<Real> wi, Planes5<Real> F, StepPar par) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const long c = box_index(g, b, t);
+    const unsigned c = box_index(g, b, t);
     Gas<Real> gas;
     gas.gam = (Real)par.gam;
     gas.invPr = (Real)par.invPr;
@@ -123,15 +189,15 @@ __global__ void __launch_bounds__(256) k_flux(Geo g, Box3 b, long sn, long st1, 
     const Real idx = Real(1.0 / par.dx);
 
     // collision time mu/p of the left cell (update.cpp:262-264), from the plane
-    const Real tau0 = tau[c];
+    const Real tau0 = ld(tau, c);
     Real WL[5], WR[5], Wl[5], Wr[5];
 #pragma unroll
     for (int s = 0; s < 5; s++) {
         const int q = slot_field<DIR>(s);
         WL[s] = (Real)st.cell(q, c);
         WR[s] = (Real)st.cell(q, c + sn);
-        Wl[s] = wi.p[q][c];
-        Wr[s] = wi.p[5 + q][c];
+        Wl[s] = ld(wi.p[q], c);
+        Wr[s] = ld(wi.p[5 + q], c);
     }
     auto slopes = [&](int side, Real Dx[5], Real Dy[5], Real Dz[5]) {
 #pragma unroll
@@ -139,30 +205,31 @@ __global__ void __launch_bounds__(256) k_flux(Geo g, Box3 b, long sn, long st1, 
             const int q = slot_field<DIR>(s);
             const int o = side == 0 ? q : 5 + q;
             Dx[s] = side == 0 ? 2 * (Wl[s] - WL[s]) * idx : 2 * (WR[s] - Wr[s]) * idx;
-            Dy[s] = Real(0.5) * (wi.p[o][c + st1] - wi.p[o][c - st1]) * idx;
-            Dz[s] = Real(0.5) * (wi.p[o][c + st2] - wi.p[o][c - st2]) * idx;
+            Dy[s] = Real(0.5) * (ld(wi.p[o], c + st1) - ld(wi.p[o], c - st1)) * idx;
+            Dz[s] = Real(0.5) * (ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2)) * idx;
         }
     };
     Real Fs[5];
     bgk_flux(gas, WL, WR, Wl, Wr, slopes, tau0, (Real)par.dt, (Real)par.dx, Fs);
 #pragma unroll
-    for (int s = 0; s < 5; s++) F.p[slot_field<DIR>(s)][c] = Fs[s];
+    for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
 
 // ---------------------------------------------------------------------------
 // Conservative update on the interior (update.cpp:455-469): the flux
 // differences are summed in the reference's order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_update_d(Geo g, Box3 b, StateD st, Planes15<double> F, double dx) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_update_d(Geo g, Box3 b, StateD S, Planes15<double> F, double dx) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const long c = box_index(g, b, t);
+    const unsigned c = box_index(g, b, t);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double* Fx = F.p[q];
         const double* Fy = F.p[5 + q];
         const double* Fz = F.p[10 + q];
-        st.w[q][c] -= (1 / dx) * (Fx[c] - Fx[c - g.s1] + Fy[c] - Fy[c - g.s0] + Fz[c] - Fz[c - 1]);
+        const double d = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
+        put(S.w[q], c, ld(S.w[q], c) - (1 / dx) * d);
     }
 }
 
@@ -178,37 +245,37 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
     hi = h2;
 }
 
-__global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM st, Planes15<float> F, float idx) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM S, Planes15<float> F, float idx) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const long c = box_index(g, b, t);
+    const unsigned c = box_index(g, b, t);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const float* Fx = F.p[q];
         const float* Fy = F.p[5 + q];
         const float* Fz = F.p[10 + q];
-        const float R = Fx[c] - Fx[c - g.s1] + Fy[c] - Fy[c - g.s0] + Fz[c] - Fz[c - 1];
-        float hi = st.hi[q][c], lo = st.lo[q][c];
+        const float R = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
+        float hi = ld(S.hi[q], c), lo = ld(S.lo[q], c);
         two_sum_add(hi, lo, -idx * R);
-        st.hi[q][c] = hi;
-        st.lo[q][c] = lo;
+        put(S.hi[q], c, hi);
+        put(S.lo[q], c, lo);
     }
 }
 
 // ---- split / merge of the mixed state --------------------------------------
-__global__ void __launch_bounds__(256) k_split(long n, const double* src, float* hi, float* lo) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_split(unsigned n, const double* src, float* hi, float* lo) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const double v = src[t];
+    const double v = ld(src, t);
     const float h = (float)v;
-    hi[t] = h;
-    lo[t] = (float)(v - (double)h);
+    put(hi, t, h);
+    put(lo, t, (float)(v - (double)h));
 }
 
-__global__ void __launch_bounds__(256) k_merge(long n, const float* hi, const float* lo, double* dst) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_merge(unsigned n, const float* hi, const float* lo, double* dst) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    dst[t] = (double)hi[t] + (double)lo[t];
+    put(dst, t, (double)ld(hi, t) + (double)ld(lo, t));
 }
 
 // ---- point-wise entry points (device arithmetic checks) --------------------
@@ -243,26 +310,21 @@ __global__ void k_weno_points(int n, const double* rec, double gam, double R, in
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const double* x = rec + (long)t * 30;
-    const Real eps = Real(1e-10);
+    StepPar par{};
+    par.gam = gam;
+    par.R = R;
+    Real v[6][6];
+    for (int q = 0; q < 5; q++)
+        for (int s = 0; s < 6; s++) v[q][s] = (Real)x[s * 5 + q];
+    for (int s = 0; s < 6; s++) {
+        const double* W = x + s * 5;
+        v[5][s] = (Real)((gam - 1) * (W[4] / W[0] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) / (W[0] * W[0])) / R);
+    }
     Real wl[5], wr[5];
-    const int nlin = energy_recon == 1 ? 5 : 4;
-    for (int q = 0; q < nlin; q++) {
-        Real c6[6];
-        for (int s = 0; s < 6; s++) c6[s] = (Real)x[s * 5 + q];
-        weno_line<RECON>(c6, eps, wl[q], wr[q]);
-    }
-    if (energy_recon == 0) {
-        Real Tc[6];
-        for (int s = 0; s < 6; s++) {
-            const double* W = x + s * 5;
-            Tc[s] = (Real)((gam - 1) * (W[4] / W[0] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) / (W[0] * W[0])) / R);
-        }
-        Real Tl, Tr;
-        weno_line<RECON>(Tc, eps, Tl, Tr);
-        const Real cR = Real(R / (gam - 1.0));
-        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
-        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
-    }
+    if (energy_recon == 1)
+        recon_one<RECON, 1>(v, Real(1e-10), par, wl, wr);
+    else
+        recon_one<RECON, 0>(v, Real(1e-10), par, wl, wr);
     for (int q = 0; q < 5; q++) {
         out[(long)t * 10 + q] = (double)wl[q];
         out[(long)t * 10 + 5 + q] = (double)wr[q];
@@ -272,7 +334,9 @@ __global__ void k_weno_points(int n, const double* rec, double gam, double R, in
 // ===========================================================================
 // launchers
 // ===========================================================================
-static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+static inline unsigned blocks_for(unsigned n, int bs) { return (n + bs - 1) / bs; }
+
+constexpr int kReconChunk = 8;
 
 // Box of interfaces for sweep `dir`: extended (WENO) or core (flux) set.
 static Box3 sweep_box(const Geo& g, int an, bool extended) {
@@ -284,23 +348,16 @@ static Box3 sweep_box(const Geo& g, int an, bool extended) {
     }
     lo[an] = 2;
     hi[an] = P[an] - 4;
-    Box3 b;
-    b.o0 = lo[0]; b.o1 = lo[1]; b.o2 = lo[2];
-    b.n0 = hi[0] - lo[0] + 1; b.n1 = hi[1] - lo[1] + 1; b.n2 = hi[2] - lo[2] + 1;
-    return b;
+    return make_box(lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1);
 }
 
-static Box3 interior_box(const Geo& g) {
-    Box3 b;
-    b.o0 = b.o1 = b.o2 = 3;
-    b.n0 = g.ny; b.n1 = g.nx; b.n2 = g.nz;
-    return b;
-}
+static Box3 interior_box(const Geo& g) { return make_box(3, 3, 3, g.ny, g.nx, g.nz); }
+static Box3 padded_box(const Geo& g) { return make_box(0, 0, 0, g.P0, g.P1, g.P2); }
 
 // axis of the sweep direction: X -> 1, Y -> 0, Z -> 2
 static inline int normal_axis(int dir) { return dir == 0 ? 1 : (dir == 1 ? 0 : 2); }
 
-static inline long axis_stride(const Geo& g, int a) { return a == 0 ? g.s0 : (a == 1 ? g.s1 : 1); }
+static inline unsigned axis_stride(const Geo& g, int a) { return a == 0 ? g.s0 : (a == 1 ? g.s1 : 1u); }
 
 // tangential axes paired with the y / z slots (update.cpp:217-245, 291-318, 380-407)
 static inline void tangential_axes(int dir, int zbug, int& t1, int& t2) {
@@ -310,24 +367,48 @@ static inline void tangential_axes(int dir, int zbug, int& t1, int& t2) {
     else { t1 = 0; t2 = 1; }
 }
 
+template <class Real, class St, int RECON, int ENERGY>
+static void launch_recon(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi, const StepPar& par) {
+    const Geo& g = h->geo;
+    const int an = normal_axis(dir);
+    const unsigned sn = axis_stride(g, an);
+    if (dir == 2) {
+        Box3 be = sweep_box(g, an, true);
+        hipLaunchKernelGGL((k_recon<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256)), dim3(256), 0,
+                           h->stream, g, be, sn, st, Tw, wi, par);
+        return;
+    }
+    // X (normal = axis 1) walks i with threads over (j, k); Y (normal = axis 0) walks j over (i, k)
+    const int P[3] = {g.P0, g.P1, g.P2};
+    const int oa = an == 1 ? 0 : 1;  // the other tangential axis
+    const int n_normal = P[an] - 5;  // [2, P-4]
+    const int chunks = (n_normal + kReconChunk - 1) / kReconChunk;
+    Box3 b = make_box(0, 0, 0, chunks, P[oa] - 4, P[2] - 4);
+    hipLaunchKernelGGL((k_recon_win<Real, St, RECON, ENERGY, kReconChunk>), dim3(blocks_for(b.count(), 256)),
+                       dim3(256), 0, h->stream, g, b, sn, axis_stride(g, oa), n_normal, st, Tw, wi, par);
+}
+
+template <class Real, class St>
+static void launch_recon_any(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi, const StepPar& par) {
+    const bool js = h->par.recon == GKM_RECON_JS, e = h->par.energy_recon == GKM_ENERGY_E;
+    if (js && e) launch_recon<Real, St, 1, 1>(h, dir, st, Tw, wi, par);
+    else if (js) launch_recon<Real, St, 1, 0>(h, dir, st, Tw, wi, par);
+    else if (e) launch_recon<Real, St, 0, 1>(h, dir, st, Tw, wi, par);
+    else launch_recon<Real, St, 0, 0>(h, dir, st, Tw, wi, par);
+}
+
 template <class Real, class St>
 static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau, Planes10<Real> wi,
                          Planes5<Real> F, const StepPar& par, hipEvent_t* ev) {
     const Geo& g = h->geo;
     const int an = normal_axis(dir);
-    const long sn = axis_stride(g, an);
+    const unsigned sn = axis_stride(g, an);
     int t1, t2;
     tangential_axes(dir, h->par.zbug_compat, t1, t2);
-    const long st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
-    Box3 be = sweep_box(g, an, true);
+    const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
     Box3 bc = sweep_box(g, an, false);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    if (h->par.recon == GKM_RECON_JS)
-        hipLaunchKernelGGL((k_recon<Real, St, 1>), dim3(blocks_for(be.count(), 256)), dim3(256), 0, h->stream, g, be,
-                           sn, st, Tw, wi, par);
-    else
-        hipLaunchKernelGGL((k_recon<Real, St, 0>), dim3(blocks_for(be.count(), 256)), dim3(256), 0, h->stream, g, be,
-                           sn, st, Tw, wi, par);
+    launch_recon_any<Real, St>(h, dir, st, Tw, wi, par);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     const int bs = h->flux_block;
     if (dir == 0)
@@ -344,14 +425,14 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
 
 void launch_ghost(Ctx* h, const StepPar* par) {
     const Geo& g = h->geo;
-    long n = (long)g.P0 * g.P1 * g.P2;
+    Box3 all = padded_box(g);
     StepPar p0 = par ? *par : StepPar{};
     if (h->par.precision == GKM_PREC_FP64) {
         Planes10<double> f{};
         StateD st;
         for (int q = 0; q < 5; q++) f.p[q] = st.w[q] = h->Wd[q];
-        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), dim3(blocks_for(n, 256)), dim3(256), 0,
-                           h->stream, g, f, st, par ? h->Td : nullptr, h->taud, p0);
+        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), dim3(blocks_for(all.count(), 256)), dim3(256),
+                           0, h->stream, g, all, f, st, par ? h->Td : nullptr, h->taud, p0);
     } else {
         Planes10<float> f{};
         StateM st;
@@ -359,8 +440,8 @@ void launch_ghost(Ctx* h, const StepPar* par) {
             f.p[q] = st.hi[q] = h->Whi[q];
             f.p[5 + q] = st.lo[q] = h->Wlo[q];
         }
-        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), dim3(blocks_for(n, 256)), dim3(256), 0,
-                           h->stream, g, f, st, par ? h->Tf : nullptr, h->tauf, p0);
+        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), dim3(blocks_for(all.count(), 256)), dim3(256), 0,
+                           h->stream, g, all, f, st, par ? h->Tf : nullptr, h->tauf, p0);
     }
 }
 
@@ -424,12 +505,12 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
 }
 
 void launch_split(Ctx* h, int q, const double* dsrc) {
-    long n = h->ncell;
+    unsigned n = (unsigned)h->ncell;
     hipLaunchKernelGGL(k_split, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, dsrc, h->Whi[q], h->Wlo[q]);
 }
 
 void launch_merge(Ctx* h, int q, double* ddst) {
-    long n = h->ncell;
+    unsigned n = (unsigned)h->ncell;
     hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, h->Whi[q], h->Wlo[q], ddst);
 }
 
