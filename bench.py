@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-realization ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=6)
     return ap.parse_args()
@@ -103,6 +105,46 @@ def cpu_baseline(args, gas_kw):
     return {"value": one, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle, {n}^3 box of the same recipe, {args.cpu_steps} steps, 1 thread",
             "omp_value": many, "omp_threads": threads, "cpu_model": cpu_model}
+
+
+def bench_ensemble(args):
+    """96 realizations (32 x 32^3, 32 x 48^3, 24 x 64^3, 8 x 96^3, Mt ~ U[0.2, 0.5]),
+    mixed precision, one batch per size, captured graphs, autotuned launch shapes;
+    E_k / Mt error of every realization against an FP64 ensemble run with the same dt."""
+    import numpy as np
+
+    from gkm_amd import ensemble as E
+    from gkm_amd import tune
+
+    reals = E.baseline_ensemble()
+    sizes = sorted({r.n for r in reals})
+    t0 = time.perf_counter()
+    tuning = tune.tune_sizes(sizes, precision="mixed", recon=args.recon)
+    t_tune = time.perf_counter() - t0
+    em = E.Ensemble(reals, precision="mixed", recon=args.recon, tuning=tuning)
+    ed = E.Ensemble(reals, precision="fp64", recon=args.recon)
+    dts = ed.timestep(args.cfl)
+    em.set_dt(dts)
+    em.step(2)
+    ed.step(2)
+    sec = em.run_timed(args.ensemble_steps)
+    sec_d = ed.run_timed(args.ensemble_steps)
+    sm, sd = em.stats(), ed.stats()
+    ek = np.array([abs(a["ek"] - b["ek"]) / abs(b["ek"]) for a, b in zip(sm, sd)])
+    mt = np.array([abs(a["mt"] - b["mt"]) / abs(b["mt"]) for a, b in zip(sm, sd)])
+    cells = em.cells
+    em.close()
+    ed.close()
+    print(f"[bench] ensemble: {1e3 * sec / args.ensemble_steps:.2f} ms/step mixed, "
+          f"{1e3 * sec_d / args.ensemble_steps:.2f} ms/step fp64", file=sys.stderr, flush=True)
+    return {"realizations": len(reals), "cells": cells, "steps": args.ensemble_steps,
+            "value": cells * args.ensemble_steps / sec, "unit": "cell-updates/s",
+            "ms_per_step": 1e3 * sec / args.ensemble_steps,
+            "fp64_value": cells * args.ensemble_steps / sec_d,
+            "ek_rel_err_max": float(ek.max()), "ek_rel_err_median": float(np.median(ek)),
+            "mt_rel_err_max": float(mt.max()),
+            "tuning": {str(n): list(v) for n, v in tuning.items()}, "tune_seconds": t_tune,
+            "tuning_gain": tune.speedup_report(sizes, precision="mixed", recon=args.recon)}
 
 
 def main():
@@ -177,6 +219,8 @@ def main():
         "mt_rel_err": mt_err,
         "cpu_baseline": None,
     }
+    if not args.no_ensemble:
+        out["ensemble"] = bench_ensemble(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, kw)
     print(json.dumps(out))

@@ -40,9 +40,6 @@ int check_params(const gkm_grid* g, const gkm_params* p) {
     if (g->nx < 3 || g->ny < 3 || g->nz < 3)
         return fail(GKM_E_ARG, "box %dx%dx%d too small: each axis needs at least 3 interior cells", g->nx, g->ny,
                     g->nz);
-    // 32-bit byte offsets address every plane: at most 2^28 padded cells
-    const double padded = (double)(g->nx + 6) * (g->ny + 6) * (g->nz + 6);
-    if (padded > 268435456.0) return fail(GKM_E_ARG, "box too large: %.0f padded cells (limit 2^28)", padded);
     if (!(g->dx > 0) || !std::isfinite(g->dx)) return fail(GKM_E_ARG, "dx must be positive");
     if (!(p->gamma > 1.0)) return fail(GKM_E_ARG, "gamma must exceed 1");
     if (!(p->R > 0) || !(p->Pr > 0) || !(p->mu0 >= 0) || !(p->T0 > 0))
@@ -55,23 +52,76 @@ int check_params(const gkm_grid* g, const gkm_params* p) {
     return GKM_OK;
 }
 
+Ctx* ctx(gkm_t* h) { return (Ctx*)h; }
+
+int check_box(Ctx* h, int box) {
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (box < 0 || box >= h->nbox) return fail(GKM_E_ARG, "box %d out of range [0, %d)", box, h->nbox);
+    return GKM_OK;
+}
+
+void drop_graph(Ctx* h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_ready = false;
+}
+
+// Upload per-box step parameters for the given dt.  They go through pinned
+// staging; the kernels read them from the device array, so a captured graph
+// stays valid when dt changes.
+int upload_pars(Ctx* h, const double* dt) {
+    bool same = h->pars_valid;
+    for (int b = 0; b < h->nbox; b++) {
+        gkm::StepPar p = gkm::step_par(h->grid, h->bpar[b], dt ? dt[b] : 0.0);
+        if (!same || std::memcmp(&p, &h->h_pars[b], sizeof p) != 0) same = false;
+        h->h_pars[b] = p;
+    }
+    if (same) return GKM_OK;
+    HIP_TRY(hipMemcpyAsync(h->d_pars, h->h_pars, sizeof(gkm::StepPar) * h->nbox, hipMemcpyHostToDevice, h->stream));
+    // the staging buffer is rewritten by the next call: wait for the copy
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->pars_valid = true;
+    return GKM_OK;
+}
+
+void free_ctx(Ctx* h) {
+    if (h->h_pars) (void)hipHostFree(h->h_pars);
+    if (h->arena) (void)hipFree(h->arena);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* gkm_last_error(void) { return g_err.c_str(); }
 
-const char* gkm_version(void) { return "gkm-mi355x 0.1 gfx950"; }
+const char* gkm_version(void) { return "gkm-mi355x 0.2 gfx950"; }
 
-int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
+int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, gkm_t** out) {
     if (!out) return fail(GKM_E_ARG, "null output handle");
     *out = nullptr;
-    int rc = check_params(grid, params);
-    if (rc) return rc;
+    if (!params || nbox < 1) return fail(GKM_E_ARG, "need nbox >= 1 parameter sets");
+    for (int b = 0; b < nbox; b++) {
+        int rc = check_params(grid, &params[b]);
+        if (rc) return rc;
+        if (params[b].recon != params[0].recon || params[b].energy_recon != params[0].energy_recon ||
+            params[b].zbug_compat != params[0].zbug_compat || params[b].precision != params[0].precision)
+            return fail(GKM_E_ARG, "box %d: recon, energy_recon, zbug_compat and precision must match box 0", b);
+    }
+    // 32-bit byte offsets address every plane of the batch
+    const double padded = (double)(grid->nx + 6) * (grid->ny + 6) * (grid->nz + 6);
+    if (padded * nbox > 268435456.0)
+        return fail(GKM_E_ARG, "batch too large: %.0f padded cells (limit 2^28)", padded * nbox);
     Ctx* h = new (std::nothrow) Ctx();
     if (!h) return fail(GKM_E_NOMEM, "host allocation failed");
     h->grid = *grid;
-    h->par = *params;
+    h->par = params[0];
+    h->bpar.assign(params, params + nbox);
+    h->nbox = nbox;
     gkm::Geo& g = h->geo;
     g.nx = grid->nx;
     g.ny = grid->ny;
@@ -81,28 +131,31 @@ int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
     g.P2 = grid->nz + 6;
     g.s1 = (unsigned)g.P2;
     g.s0 = (unsigned)g.P1 * (unsigned)g.P2;
-    h->ncell = (long)g.P0 * g.P1 * g.P2;
-    h->flux_block = 256;
+    g.ncell = (unsigned)g.P0 * g.s0;
+    h->ncell = (long)g.ncell;
     hipError_t e = hipGetDevice(&h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        delete h;
+        h->stream = nullptr;
+        free_ctx(h);
         return fail(GKM_E_HIP, "no usable HIP device: %s", hipGetErrorString(e));
     }
-    const size_t n = (size_t)h->ncell;
+    const size_t n = (size_t)h->ncell * nbox;
     const size_t align = 256;
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
-    const bool fp64 = params->precision == GKM_PREC_FP64;
+    const bool fp64 = params[0].precision == GKM_PREC_FP64;
     const size_t plane = up(n * (fp64 ? sizeof(double) : sizeof(float)));
     const int nplanes = fp64 ? (5 + 10 + 15 + 2) : (10 + 10 + 15 + 2);
     h->red_blocks = 2048;
     const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
-    h->arena_bytes = plane * nplanes + red;
+    const size_t pars = up(sizeof(gkm::StepPar) * nbox);
+    h->arena_bytes = plane * nplanes + red + pars;
     e = hipMalloc(&h->arena, h->arena_bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_pars, sizeof(gkm::StepPar) * nbox, hipHostMallocDefault);
     if (e != hipSuccess) {
-        (void)hipStreamDestroy(h->stream);
-        delete h;
-        return fail(GKM_E_NOMEM, "hipMalloc of %zu bytes failed: %s", (size_t)0 + n * nplanes, hipGetErrorString(e));
+        const size_t bytes = h->arena_bytes;
+        free_ctx(h);
+        return fail(GKM_E_NOMEM, "allocation of %zu device bytes failed: %s", bytes, hipGetErrorString(e));
     }
     char* base = (char*)h->arena;
     auto take = [&](size_t b) {
@@ -126,63 +179,85 @@ int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
     }
     h->red_part = (double*)take(up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA));
     h->red_out = (double*)take(up(sizeof(double) * 64));
+    h->d_pars = (gkm::StepPar*)take(pars);
     // interface and flux planes start at zero so that unused border entries are defined
     e = hipMemsetAsync(h->arena, 0, h->arena_bytes, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
-        (void)hipFree(h->arena);
-        (void)hipStreamDestroy(h->stream);
-        delete h;
+        free_ctx(h);
         return fail(GKM_E_HIP, "initial clear failed: %s", hipGetErrorString(e));
+    }
+    int rc = upload_pars(h, nullptr);
+    if (rc) {
+        free_ctx(h);
+        return rc;
     }
     *out = (gkm_t*)h;
     return GKM_OK;
 }
 
-int gkm_destroy(gkm_t* hh) {
-    Ctx* h = (Ctx*)hh;
-    if (!h) return GKM_OK;
-    (void)hipStreamSynchronize(h->stream);
-    if (h->events_made)
-        for (auto& ev : h->ev_step) (void)hipEventDestroy(ev);
-    (void)hipFree(h->arena);
-    (void)hipStreamDestroy(h->stream);
-    delete h;
+int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
+    return gkm_create_batch(grid, params, 1, out);
+}
+
+int gkm_batch_size(gkm_t* hh, int* nbox) {
+    Ctx* h = ctx(hh);
+    if (!h || !nbox) return fail(GKM_E_ARG, "null argument");
+    *nbox = h->nbox;
     return GKM_OK;
 }
 
-int gkm_set_state(gkm_t* hh, const double* const W[5]) {
-    Ctx* h = (Ctx*)hh;
-    if (!h || !W) return fail(GKM_E_ARG, "null handle or state");
+int gkm_destroy(gkm_t* hh) {
+    Ctx* h = ctx(hh);
+    if (!h) return GKM_OK;
+    (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    if (h->events_made)
+        for (auto& ev : h->ev_step) (void)hipEventDestroy(ev);
+    free_ctx(h);
+    return GKM_OK;
+}
+
+int gkm_set_state_box(gkm_t* hh, int box, const double* const W[5]) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!W) return fail(GKM_E_ARG, "null state");
     for (int q = 0; q < 5; q++)
         if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
     const size_t bytes = sizeof(double) * (size_t)h->ncell;
+    const size_t off = (size_t)box * h->ncell;
     if (h->par.precision == GKM_PREC_FP64) {
-        for (int q = 0; q < 5; q++) HIP_TRY(hipMemcpyAsync(h->Wd[q], W[q], bytes, hipMemcpyHostToDevice, h->stream));
+        for (int q = 0; q < 5; q++)
+            HIP_TRY(hipMemcpyAsync(h->Wd[q] + off, W[q], bytes, hipMemcpyHostToDevice, h->stream));
     } else {
-        // stage through the (float) flux planes: 15 float planes hold >= 1 double plane
+        // stage through the (float) flux planes: 15 float planes of the batch hold one double plane
         double* stage = (double*)h->Ff[0];
         for (int q = 0; q < 5; q++) {
             HIP_TRY(hipMemcpyAsync(stage, W[q], bytes, hipMemcpyHostToDevice, h->stream));
-            gkm::launch_split(h, q, stage);
+            gkm::launch_split(h, box, q, stage);
         }
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
     return GKM_OK;
 }
 
-int gkm_get_state(gkm_t* hh, double* const W[5]) {
-    Ctx* h = (Ctx*)hh;
-    if (!h || !W) return fail(GKM_E_ARG, "null handle or state");
+int gkm_get_state_box(gkm_t* hh, int box, double* const W[5]) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!W) return fail(GKM_E_ARG, "null state");
     for (int q = 0; q < 5; q++)
         if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
     const size_t bytes = sizeof(double) * (size_t)h->ncell;
+    const size_t off = (size_t)box * h->ncell;
     if (h->par.precision == GKM_PREC_FP64) {
-        for (int q = 0; q < 5; q++) HIP_TRY(hipMemcpyAsync(W[q], h->Wd[q], bytes, hipMemcpyDeviceToHost, h->stream));
+        for (int q = 0; q < 5; q++)
+            HIP_TRY(hipMemcpyAsync(W[q], h->Wd[q] + off, bytes, hipMemcpyDeviceToHost, h->stream));
     } else {
         double* stage = (double*)h->Ff[0];
         for (int q = 0; q < 5; q++) {
-            gkm::launch_merge(h, q, stage);
+            gkm::launch_merge(h, box, q, stage);
             HIP_TRY(hipMemcpyAsync(W[q], stage, bytes, hipMemcpyDeviceToHost, h->stream));
         }
     }
@@ -190,19 +265,25 @@ int gkm_get_state(gkm_t* hh, double* const W[5]) {
     return GKM_OK;
 }
 
+int gkm_set_state(gkm_t* h, const double* const W[5]) { return gkm_set_state_box(h, 0, W); }
+int gkm_get_state(gkm_t* h, double* const W[5]) { return gkm_get_state_box(h, 0, W); }
+
 int gkm_fill_ghosts(gkm_t* hh) {
-    Ctx* h = (Ctx*)hh;
+    Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
-    gkm::launch_ghost(h, nullptr);
+    gkm::launch_ghost(h, false);
     HIP_TRY(hipGetLastError());
     return GKM_OK;
 }
 
-int gkm_step(gkm_t* hh, double dt, int nsteps) {
-    Ctx* h = (Ctx*)hh;
-    if (!h) return fail(GKM_E_ARG, "null handle");
-    if (!(dt > 0) || !std::isfinite(dt)) return fail(GKM_E_ARG, "dt must be positive and finite");
+int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
+    Ctx* h = ctx(hh);
+    if (!h || !dt) return fail(GKM_E_ARG, "null handle or dt");
+    for (int b = 0; b < h->nbox; b++)
+        if (!(dt[b] > 0) || !std::isfinite(dt[b])) return fail(GKM_E_ARG, "dt[%d] must be positive and finite", b);
     if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
+    int rc = upload_pars(h, dt);
+    if (rc) return rc;
     if (h->timing && !h->events_made) {
         for (auto& ev : h->ev_step) HIP_TRY(hipEventCreate(&ev));
         h->events_made = true;
@@ -210,30 +291,75 @@ int gkm_step(gkm_t* hh, double dt, int nsteps) {
     h->timed_steps = 0;
     for (int s = 0; s < nsteps; s++) {
         const bool timed = h->timing && s < gkm::kTimedSteps;
-        gkm::launch_step(h, dt, timed, s);
-        if (timed) h->timed_steps = s + 1;
+        if (timed) {
+            gkm::launch_step(h, h->ev_step + s * 12);
+            h->timed_steps = s + 1;
+        } else if (h->graph_enabled) {
+            if (!h->graph_ready) {
+                HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                gkm::launch_step(h, nullptr);
+                hipGraph_t gr = nullptr;
+                HIP_TRY(hipStreamEndCapture(h->stream, &gr));
+                h->graph = gr;
+                HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+                h->graph_ready = true;
+            }
+            HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+        } else {
+            gkm::launch_step(h, nullptr);
+        }
     }
-    gkm::launch_ghost(h, nullptr);
+    gkm::launch_ghost(h, false);
     HIP_TRY(hipGetLastError());
     return GKM_OK;
 }
 
+int gkm_step(gkm_t* hh, double dt, int nsteps) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (h->nbox != 1) return fail(GKM_E_ARG, "handle holds %d boxes: use gkm_step_batch", h->nbox);
+    return gkm_step_batch(hh, &dt, nsteps);
+}
+
+int gkm_set_graph(gkm_t* hh, int enabled) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->graph_enabled = enabled != 0;
+    if (!h->graph_enabled) drop_graph(h);
+    return GKM_OK;
+}
+
+int gkm_set_tuning(gkm_t* hh, int flux_block, int recon_chunk) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (flux_block != 64 && flux_block != 128 && flux_block != 256)
+        return fail(GKM_E_ARG, "flux_block must be 64, 128 or 256 (got %d)", flux_block);
+    if (recon_chunk != 4 && recon_chunk != 8 && recon_chunk != 16)
+        return fail(GKM_E_ARG, "recon_chunk must be 4, 8 or 16 (got %d)", recon_chunk);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->flux_block = flux_block;
+    h->recon_chunk = recon_chunk;
+    drop_graph(h);
+    return GKM_OK;
+}
+
 int gkm_sync(gkm_t* hh) {
-    Ctx* h = (Ctx*)hh;
+    Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
     HIP_TRY(hipStreamSynchronize(h->stream));
     return GKM_OK;
 }
 
 int gkm_set_timing(gkm_t* hh, int enabled) {
-    Ctx* h = (Ctx*)hh;
+    Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
     h->timing = enabled != 0;
     return GKM_OK;
 }
 
 int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
-    Ctx* h = (Ctx*)hh;
+    Ctx* h = ctx(hh);
     if (!h || !ms) return fail(GKM_E_ARG, "null argument");
     for (int i = 0; i < 4; i++) ms[i] = -1.0;
     if (nsteps) *nsteps = h->timed_steps;
@@ -261,24 +387,30 @@ int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
     return GKM_OK;
 }
 
-int gkm_compute_dt(gkm_t* hh, double cfl, double* dt) {
-    Ctx* h = (Ctx*)hh;
-    if (!h || !dt) return fail(GKM_E_ARG, "null argument");
+int gkm_compute_dt_box(gkm_t* hh, int box, double cfl, double* dt) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!dt) return fail(GKM_E_ARG, "null argument");
     if (!(cfl > 0)) return fail(GKM_E_ARG, "cfl must be positive");
     double a[gkm::kStatA];
-    HIP_TRY(gkm::launch_stats_a(h, a));
+    HIP_TRY(gkm::launch_stats_a(h, box, a));
     const double mx = a[27];
     if (!(mx > 0) || !std::isfinite(mx)) return fail(GKM_E_NONPHYSICAL, "signal speed %g is not positive/finite", mx);
     *dt = cfl * h->grid.dx / mx;
     return GKM_OK;
 }
 
-int gkm_stats(gkm_t* hh, gkm_stats_t* out) {
-    Ctx* h = (Ctx*)hh;
-    if (!h || !out) return fail(GKM_E_ARG, "null argument");
-    gkm::launch_ghost(h, nullptr);
+int gkm_compute_dt(gkm_t* h, double cfl, double* dt) { return gkm_compute_dt_box(h, 0, cfl, dt); }
+
+int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!out) return fail(GKM_E_ARG, "null argument");
+    gkm::launch_ghost(h, false);
     double a[gkm::kStatA];
-    HIP_TRY(gkm::launch_stats_a(h, a));
+    HIP_TRY(gkm::launch_stats_a(h, box, a));
     const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
     gkm_stats_t s;
     std::memset(&s, 0, sizeof s);
@@ -314,7 +446,7 @@ int gkm_stats(gkm_t* hh, gkm_stats_t* out) {
     s.kurt_w = s.dwdz4 / (s.dwdz2 * s.dwdz2);
     const double m[4] = {s.rho_mean, s.p_mean, s.T_mean, s.v_mean};
     double b[gkm::kStatB];
-    HIP_TRY(gkm::launch_stats_b(h, m, b));
+    HIP_TRY(gkm::launch_stats_b(h, box, m, b));
     s.rho_rms = std::sqrt(b[0] / N);
     s.p_rms = std::sqrt(b[1] / N);
     s.T_rms = std::sqrt(b[2] / N);
@@ -324,6 +456,8 @@ int gkm_stats(gkm_t* hh, gkm_stats_t* out) {
         return fail(GKM_E_NONPHYSICAL, "non-physical state: <rho>=%g <p>=%g", s.rho_mean, s.p_mean);
     return GKM_OK;
 }
+
+int gkm_stats(gkm_t* h, gkm_stats_t* out) { return gkm_stats_box(h, 0, out); }
 
 static int points_common(const gkm_params* p, int n, const double* in, size_t in_per, double* out, size_t out_per,
                          double** din, double** dout, hipStream_t* s) {

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/gkm.h"
 #include "gkm_kernels.h"
 
@@ -12,12 +14,24 @@ constexpr int kTimedSteps = 64;  // steps per gkm_step call that carry HIP event
 
 struct Ctx {
     gkm_grid grid;
-    gkm_params par;
+    gkm_params par;                 // box 0's parameters (shared options)
+    std::vector<gkm_params> bpar;   // per-box parameters
     Geo geo;
-    long ncell;  // padded cells per plane
+    long ncell;  // padded cells of one box
+    int nbox = 1;
     int device;
     hipStream_t stream;
-    int flux_block;
+    int flux_block = 256;
+    int recon_chunk = 8;
+    // per-box step parameters: device array read by the kernels, pinned staging
+    StepPar* d_pars = nullptr;
+    StepPar* h_pars = nullptr;
+    bool pars_valid = false;
+    // captured step
+    bool graph_enabled = false;
+    bool graph_ready = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     // FP64 form
     double* Wd[5] = {};
     double* WId[10] = {};
@@ -45,11 +59,11 @@ struct Ctx {
     bool events_made = false;
 };
 
-StepPar step_par(const Ctx* h, double dt);
-void launch_ghost(Ctx* h, const StepPar* par);  // par == nullptr: ghosts only
-void launch_step(Ctx* h, double dt, bool timed, int step_index);
-void launch_split(Ctx* h, int q, const double* dsrc);
-void launch_merge(Ctx* h, int q, double* ddst);
+StepPar step_par(const gkm_grid& grid, const gkm_params& p, double dt);
+void launch_ghost(Ctx* h, bool prim);  // prim = false: ghosts only
+void launch_step(Ctx* h, hipEvent_t* ev);  // ev: 12 events or nullptr
+void launch_split(Ctx* h, int box, int q, const double* dsrc);
+void launch_merge(Ctx* h, int box, int q, double* ddst);
 void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,
                         double* out);
 void launch_weno_points(hipStream_t s, int prec, int recon, int n, const double* rec, double gam, double R,
@@ -58,7 +72,7 @@ void launch_weno_points(hipStream_t s, int prec, int recon, int n, const double*
 // statistics: number of accumulated quantities of each pass
 constexpr int kStatA = 28;  // sums + one max (slot 27)
 constexpr int kStatB = 4;
-hipError_t launch_stats_a(Ctx* h, double* host_out /*kStatA*/);
-hipError_t launch_stats_b(Ctx* h, const double means[4], double* host_out /*kStatB*/);
+hipError_t launch_stats_a(Ctx* h, int box, double* host_out /*kStatA*/);
+hipError_t launch_stats_b(Ctx* h, int box, const double means[4], double* host_out /*kStatB*/);
 
 }  // namespace gkm

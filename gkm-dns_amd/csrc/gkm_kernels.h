@@ -21,6 +21,7 @@ struct Geo {
     int P0, P1, P2;
     unsigned s0, s1;  // s2 = 1
     int nx, ny, nz;
+    unsigned ncell;   // padded cells of one box; box b's planes start at b * ncell
 };
 
 // Division by a run-time invariant divisor d >= 1 for dividends t < 2^31:

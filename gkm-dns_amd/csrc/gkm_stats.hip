@@ -175,31 +175,33 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(int nb, const doub
     }
 }
 
-static StatPar stat_par(const Ctx* h) {
+static StatPar stat_par(const Ctx* h, int box) {
+    const gkm_params& q = h->bpar[box];
     StatPar p;
-    p.gam = h->par.gamma;
-    p.R = h->par.R;
-    p.mu0 = h->par.mu0;
-    p.T0 = h->par.T0;
-    p.visc_exp = h->par.visc_exp;
+    p.gam = q.gamma;
+    p.R = q.R;
+    p.mu0 = q.mu0;
+    p.T0 = q.T0;
+    p.visc_exp = q.visc_exp;
     p.dx = h->grid.dx;
-    p.visc_law = h->par.visc_law;
+    p.visc_law = q.visc_law;
     return p;
 }
 
-hipError_t launch_stats_a(Ctx* h, double* host_out) {
+hipError_t launch_stats_a(Ctx* h, int box, double* host_out) {
     const Geo& g = h->geo;
-    const StatPar p = stat_par(h);
+    const StatPar p = stat_par(h, box);
     const int nb = h->red_blocks;
+    const size_t off = (size_t)box * h->ncell;
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
-        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
         hipLaunchKernelGGL(k_stats_a<StateD>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, h->red_part);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
-            st.hi[q] = h->Whi[q];
-            st.lo[q] = h->Wlo[q];
+            st.hi[q] = h->Whi[q] + off;
+            st.lo[q] = h->Wlo[q] + off;
         }
         hipLaunchKernelGGL(k_stats_a<StateM>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, h->red_part);
     }
@@ -210,20 +212,21 @@ hipError_t launch_stats_a(Ctx* h, double* host_out) {
     return hipStreamSynchronize(h->stream);
 }
 
-hipError_t launch_stats_b(Ctx* h, const double m[4], double* host_out) {
+hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) {
     const Geo& g = h->geo;
-    const StatPar p = stat_par(h);
+    const StatPar p = stat_par(h, box);
     const int nb = h->red_blocks;
+    const size_t off = (size_t)box * h->ncell;
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
-        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
         hipLaunchKernelGGL(k_stats_b<StateD>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
                            m[3], h->red_part);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
-            st.hi[q] = h->Whi[q];
-            st.lo[q] = h->Wlo[q];
+            st.hi[q] = h->Whi[q] + off;
+            st.lo[q] = h->Wlo[q] + off;
         }
         hipLaunchKernelGGL(k_stats_b<StateM>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
                            m[3], h->red_part);

@@ -28,11 +28,13 @@ __device__ __forceinline__ double viscosity(const StepPar& par, double T) {
 
 template <int NPL, class T, class Real, class St>
 __global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Box3 all, Planes10<T> f, St st, Real* Tw, Real* tau,
-                                                    StepPar par) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= all.count()) return;
-    const unsigned r = fdiv(t, all.f2);
-    const int k = (int)(t - r * (unsigned)all.n2);
+                                                    const StepPar* __restrict__ pars) {
+    const unsigned tl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tl >= all.count()) return;
+    const unsigned base = blockIdx.y * g.ncell;
+    const unsigned t = base + tl;
+    const unsigned r = fdiv(tl, all.f2);
+    const int k = (int)(tl - r * (unsigned)all.n2);
     const int j = (int)fdiv(r, all.f1);
     const int i = (int)(r - (unsigned)j * (unsigned)all.n1);
     const bool gj = j < 3 || j >= g.ny + 3, gi = i < 3 || i >= g.nx + 3, gk = k < 3 || k >= g.nz + 3;
@@ -41,11 +43,12 @@ __global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Box3 all, Planes10<T>
         const int js = j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
         const int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
         const int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
-        src = (unsigned)js * g.s0 + (unsigned)is * g.s1 + (unsigned)ks;
+        src = base + (unsigned)js * g.s0 + (unsigned)is * g.s1 + (unsigned)ks;
 #pragma unroll
         for (int q = 0; q < NPL; q++) put(f.p[q], t, ld(f.p[q], src));
     }
     if (!Tw) return;
+    const StepPar& par = pars[blockIdx.y];
     const double r0 = st.get(0, src), m1 = st.get(1, src), m2 = st.get(2, src), m3 = st.get(3, src),
                  E = st.get(4, src);
     put(Tw, t, (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R));
@@ -84,10 +87,11 @@ __device__ __forceinline__ void recon_one(const Real (&v)[6][6], Real eps, const
 // their six-cell stencils through the cache.
 template <class Real, class St, int RECON, int ENERGY>
 __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st, const Real* __restrict__ Tw,
-                                               Planes10<Real> wi, StepPar par) {
+                                               Planes10<Real> wi, const StepPar* __restrict__ pars) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const unsigned c = box_index(g, b, t);
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
     Real v[6][6];
     constexpr int nlin = ENERGY == 1 ? 5 : 4;
 #pragma unroll
@@ -112,16 +116,18 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st
 // coalescing.  Box axes: n0 = chunks, n1 = the other tangential axis, n2 = z.
 template <class Real, class St, int RECON, int ENERGY, int CHUNK>
 __global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, unsigned sa, int n_normal, St st,
-                                                   const Real* __restrict__ Tw, Planes10<Real> wi, StepPar par) {
+                                                   const Real* __restrict__ Tw, Planes10<Real> wi,
+                                                   const StepPar* __restrict__ pars) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
+    const StepPar& par = pars[blockIdx.y];
     const unsigned r = fdiv(t, b.f2);
     const unsigned k = t - r * (unsigned)b.n2;
     const unsigned chunk = fdiv(r, b.f1);
     const unsigned a = r - chunk * (unsigned)b.n1;
     const int first = (int)chunk * CHUNK;                 // normal index offset from 2
     const int count = min(CHUNK, n_normal - first);
-    unsigned c = (unsigned)(first + 2) * sn + (a + 2) * sa + (k + 2);
+    unsigned c = blockIdx.y * g.ncell + (unsigned)(first + 2) * sn + (a + 2) * sa + (k + 2);
     constexpr bool tmode = ENERGY == 0;
     constexpr int nlin = tmode ? 4 : 5;
     Real v[6][6];
@@ -176,10 +182,11 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 template <class Real, class St, int DIR>
 __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
-           Planes10<Real> wi, Planes5<Real> F, StepPar par) {
+           Planes10<Real> wi, Planes5<Real> F, const StepPar* __restrict__ pars) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const unsigned c = box_index(g, b, t);
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
     Gas<Real> gas;
     gas.gam = (Real)par.gam;
     gas.invPr = (Real)par.invPr;
@@ -222,7 +229,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
 __global__ void __launch_bounds__(256) k_update_d(Geo g, Box3 b, StateD S, Planes15<double> F, double dx) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const unsigned c = box_index(g, b, t);
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double* Fx = F.p[q];
@@ -248,7 +255,7 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
 __global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM S, Planes15<float> F, float idx) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const unsigned c = box_index(g, b, t);
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const float* Fx = F.p[q];
@@ -336,8 +343,6 @@ __global__ void k_weno_points(int n, const double* rec, double gam, double R, in
 // ===========================================================================
 static inline unsigned blocks_for(unsigned n, int bs) { return (n + bs - 1) / bs; }
 
-constexpr int kReconChunk = 8;
-
 // Box of interfaces for sweep `dir`: extended (WENO) or core (flux) set.
 static Box3 sweep_box(const Geo& g, int an, bool extended) {
     int P[3] = {g.P0, g.P1, g.P2};
@@ -367,39 +372,52 @@ static inline void tangential_axes(int dir, int zbug, int& t1, int& t2) {
     else { t1 = 0; t2 = 1; }
 }
 
+template <class Real, class St, int RECON, int ENERGY, int CHUNK>
+static void launch_recon_win(Ctx* h, const Box3& b, unsigned sn, unsigned sa, int n_normal, St st, const Real* Tw,
+                             Planes10<Real> wi) {
+    hipLaunchKernelGGL((k_recon_win<Real, St, RECON, ENERGY, CHUNK>), dim3(blocks_for(b.count(), 256), h->nbox),
+                       dim3(256), 0, h->stream, h->geo, b, sn, sa, n_normal, st, Tw, wi, h->d_pars);
+}
+
 template <class Real, class St, int RECON, int ENERGY>
-static void launch_recon(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi, const StepPar& par) {
+static void launch_recon(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi) {
     const Geo& g = h->geo;
     const int an = normal_axis(dir);
     const unsigned sn = axis_stride(g, an);
     if (dir == 2) {
         Box3 be = sweep_box(g, an, true);
-        hipLaunchKernelGGL((k_recon<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256)), dim3(256), 0,
-                           h->stream, g, be, sn, st, Tw, wi, par);
+        hipLaunchKernelGGL((k_recon<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256), h->nbox), dim3(256),
+                           0, h->stream, g, be, sn, st, Tw, wi, h->d_pars);
         return;
     }
     // X (normal = axis 1) walks i with threads over (j, k); Y (normal = axis 0) walks j over (i, k)
     const int P[3] = {g.P0, g.P1, g.P2};
     const int oa = an == 1 ? 0 : 1;  // the other tangential axis
     const int n_normal = P[an] - 5;  // [2, P-4]
-    const int chunks = (n_normal + kReconChunk - 1) / kReconChunk;
+    const int chunk = h->recon_chunk;
+    const int chunks = (n_normal + chunk - 1) / chunk;
     Box3 b = make_box(0, 0, 0, chunks, P[oa] - 4, P[2] - 4);
-    hipLaunchKernelGGL((k_recon_win<Real, St, RECON, ENERGY, kReconChunk>), dim3(blocks_for(b.count(), 256)),
-                       dim3(256), 0, h->stream, g, b, sn, axis_stride(g, oa), n_normal, st, Tw, wi, par);
+    const unsigned sa = axis_stride(g, oa);
+    if (chunk == 4)
+        launch_recon_win<Real, St, RECON, ENERGY, 4>(h, b, sn, sa, n_normal, st, Tw, wi);
+    else if (chunk == 16)
+        launch_recon_win<Real, St, RECON, ENERGY, 16>(h, b, sn, sa, n_normal, st, Tw, wi);
+    else
+        launch_recon_win<Real, St, RECON, ENERGY, 8>(h, b, sn, sa, n_normal, st, Tw, wi);
 }
 
 template <class Real, class St>
-static void launch_recon_any(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi, const StepPar& par) {
+static void launch_recon_any(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> wi) {
     const bool js = h->par.recon == GKM_RECON_JS, e = h->par.energy_recon == GKM_ENERGY_E;
-    if (js && e) launch_recon<Real, St, 1, 1>(h, dir, st, Tw, wi, par);
-    else if (js) launch_recon<Real, St, 1, 0>(h, dir, st, Tw, wi, par);
-    else if (e) launch_recon<Real, St, 0, 1>(h, dir, st, Tw, wi, par);
-    else launch_recon<Real, St, 0, 0>(h, dir, st, Tw, wi, par);
+    if (js && e) launch_recon<Real, St, 1, 1>(h, dir, st, Tw, wi);
+    else if (js) launch_recon<Real, St, 1, 0>(h, dir, st, Tw, wi);
+    else if (e) launch_recon<Real, St, 0, 1>(h, dir, st, Tw, wi);
+    else launch_recon<Real, St, 0, 0>(h, dir, st, Tw, wi);
 }
 
 template <class Real, class St>
 static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau, Planes10<Real> wi,
-                         Planes5<Real> F, const StepPar& par, hipEvent_t* ev) {
+                         Planes5<Real> F, hipEvent_t* ev) {
     const Geo& g = h->geo;
     const int an = normal_axis(dir);
     const unsigned sn = axis_stride(g, an);
@@ -408,31 +426,32 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
     Box3 bc = sweep_box(g, an, false);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_recon_any<Real, St>(h, dir, st, Tw, wi, par);
+    launch_recon_any<Real, St>(h, dir, st, Tw, wi);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     const int bs = h->flux_block;
+    const dim3 grid(blocks_for(bc.count(), bs), h->nbox);
     if (dir == 0)
-        hipLaunchKernelGGL((k_flux<Real, St, 0>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, tau, wi, F, par);
+        hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+                           h->d_pars);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_flux<Real, St, 1>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, tau, wi, F, par);
+        hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+                           h->d_pars);
     else
-        hipLaunchKernelGGL((k_flux<Real, St, 2>), dim3(blocks_for(bc.count(), bs)), dim3(bs), 0, h->stream, g, bc, sn,
-                           st1, st2, st, tau, wi, F, par);
+        hipLaunchKernelGGL((k_flux<Real, St, 2>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+                           h->d_pars);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
-void launch_ghost(Ctx* h, const StepPar* par) {
+void launch_ghost(Ctx* h, bool prim) {
     const Geo& g = h->geo;
     Box3 all = padded_box(g);
-    StepPar p0 = par ? *par : StepPar{};
+    const dim3 grid(blocks_for(all.count(), 256), h->nbox);
     if (h->par.precision == GKM_PREC_FP64) {
         Planes10<double> f{};
         StateD st;
         for (int q = 0; q < 5; q++) f.p[q] = st.w[q] = h->Wd[q];
-        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), dim3(blocks_for(all.count(), 256)), dim3(256),
-                           0, h->stream, g, all, f, st, par ? h->Td : nullptr, h->taud, p0);
+        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), grid, dim3(256), 0, h->stream, g, all, f, st,
+                           prim ? h->Td : nullptr, h->taud, h->d_pars);
     } else {
         Planes10<float> f{};
         StateM st;
@@ -440,35 +459,34 @@ void launch_ghost(Ctx* h, const StepPar* par) {
             f.p[q] = st.hi[q] = h->Whi[q];
             f.p[5 + q] = st.lo[q] = h->Wlo[q];
         }
-        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), dim3(blocks_for(all.count(), 256)), dim3(256), 0,
-                           h->stream, g, all, f, st, par ? h->Tf : nullptr, h->tauf, p0);
+        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), grid, dim3(256), 0, h->stream, g, all, f, st,
+                           prim ? h->Tf : nullptr, h->tauf, h->d_pars);
     }
 }
 
-StepPar step_par(const Ctx* h, double dt) {
+StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
     StepPar p;
-    p.gam = h->par.gamma;
-    p.R = h->par.R;
-    p.invPr = 1.0 / h->par.Pr;
-    p.mu0 = h->par.mu0;
-    p.T0 = h->par.T0;
-    p.visc_exp = h->par.visc_exp;
+    p.gam = pp.gamma;
+    p.R = pp.R;
+    p.invPr = 1.0 / pp.Pr;
+    p.mu0 = pp.mu0;
+    p.T0 = pp.T0;
+    p.visc_exp = pp.visc_exp;
     p.dt = dt;
-    p.dx = h->grid.dx;
-    p.K = h->par.K;
-    p.visc_law = h->par.visc_law;
-    p.energy_recon = h->par.energy_recon;
+    p.dx = grid.dx;
+    p.K = pp.K;
+    p.visc_law = pp.visc_law;
+    p.energy_recon = pp.energy_recon;
     return p;
 }
 
-void launch_step(Ctx* h, double dt, bool timed, int step_index) {
+void launch_step(Ctx* h, hipEvent_t* ev) {
     const Geo& g = h->geo;
-    const StepPar par = step_par(h, dt);
-    hipEvent_t* ev = timed ? h->ev_step + step_index * 12 : nullptr;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_ghost(h, &par);
+    launch_ghost(h, true);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     Box3 bi = interior_box(g);
+    const dim3 ugrid(blocks_for(bi.count(), 256), h->nbox);
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
@@ -477,12 +495,11 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
         for (int dir = 0; dir < 3; dir++) {
             Planes5<double> F;
             for (int q = 0; q < 5; q++) F.p[q] = h->Fd[5 * dir + q];
-            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, F, ev ? ev + 1 + 3 * dir : nullptr);
         }
         Planes15<double> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Fd[q];
-        hipLaunchKernelGGL(k_update_d, dim3(blocks_for(bi.count(), 256)), dim3(256), 0, h->stream, g, bi, st, F,
-                           h->grid.dx);
+        hipLaunchKernelGGL(k_update_d, ugrid, dim3(256), 0, h->stream, g, bi, st, F, h->grid.dx);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
@@ -494,24 +511,27 @@ void launch_step(Ctx* h, double dt, bool timed, int step_index) {
         for (int dir = 0; dir < 3; dir++) {
             Planes5<float> F;
             for (int q = 0; q < 5; q++) F.p[q] = h->Ff[5 * dir + q];
-            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, F, par, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, F, ev ? ev + 1 + 3 * dir : nullptr);
         }
         Planes15<float> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Ff[q];
-        hipLaunchKernelGGL(k_update_m, dim3(blocks_for(bi.count(), 256)), dim3(256), 0, h->stream, g, bi, st, F,
-                           (float)(1.0 / h->grid.dx));
+        hipLaunchKernelGGL(k_update_m, ugrid, dim3(256), 0, h->stream, g, bi, st, F, (float)(1.0 / h->grid.dx));
     }
     if (ev) (void)hipEventRecord(ev[11], h->stream);
 }
 
-void launch_split(Ctx* h, int q, const double* dsrc) {
-    unsigned n = (unsigned)h->ncell;
-    hipLaunchKernelGGL(k_split, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, dsrc, h->Whi[q], h->Wlo[q]);
+void launch_split(Ctx* h, int box, int q, const double* dsrc) {
+    unsigned n = h->geo.ncell;
+    const size_t off = (size_t)box * n;
+    hipLaunchKernelGGL(k_split, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, dsrc, h->Whi[q] + off,
+                       h->Wlo[q] + off);
 }
 
-void launch_merge(Ctx* h, int q, double* ddst) {
-    unsigned n = (unsigned)h->ncell;
-    hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, h->Whi[q], h->Wlo[q], ddst);
+void launch_merge(Ctx* h, int box, int q, double* ddst) {
+    unsigned n = h->geo.ncell;
+    const size_t off = (size_t)box * n;
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, h->Whi[q] + off,
+                       h->Wlo[q] + off, ddst);
 }
 
 void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,

@@ -30,6 +30,8 @@ EXPORTS = (
     "gkm_create", "gkm_destroy", "gkm_set_state", "gkm_get_state", "gkm_fill_ghosts", "gkm_step",
     "gkm_compute_dt", "gkm_stats", "gkm_sync", "gkm_set_timing", "gkm_last_timing", "gkm_flux_points",
     "gkm_weno_points", "gkm_last_error", "gkm_version",
+    "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
+    "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning",
 )
 
 
@@ -93,6 +95,15 @@ def load() -> C.CDLL:
     L.gkm_last_timing.argtypes = [vp, DP, C.POINTER(C.c_int)]
     L.gkm_flux_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
     L.gkm_weno_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
+    L.gkm_create_batch.argtypes = [C.POINTER(GkmGrid), C.POINTER(GkmParams), C.c_int, C.POINTER(vp)]
+    L.gkm_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gkm_set_state_box.argtypes = [vp, C.c_int, C.POINTER(DP)]
+    L.gkm_get_state_box.argtypes = [vp, C.c_int, C.POINTER(DP)]
+    L.gkm_step_batch.argtypes = [vp, DP, C.c_int]
+    L.gkm_compute_dt_box.argtypes = [vp, C.c_int, C.c_double, DP]
+    L.gkm_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmStats)]
+    L.gkm_set_graph.argtypes = [vp, C.c_int]
+    L.gkm_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.gkm_last_error.restype = C.c_char_p
     L.gkm_version.restype = C.c_char_p
     for name in EXPORTS:

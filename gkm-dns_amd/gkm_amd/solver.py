@@ -193,6 +193,14 @@ class Solver:
         self.info.history.append((self.info.t, s))
         return self.info.history
 
+    # -- launch options ---------------------------------------------------------------
+    def set_graph(self, on: bool = True) -> None:
+        """Capture one step in a hipGraph and replay it (gkm_set_graph)."""
+        _abi.check(self._L.gkm_set_graph(self._h, int(on)))
+
+    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
+        _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
+
     # -- timing -----------------------------------------------------------------------
     def set_timing(self, on: bool = True) -> None:
         _abi.check(self._L.gkm_set_timing(self._h, int(on)))
@@ -219,6 +227,94 @@ class Solver:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Batch:
+    """nbox independent boxes of one grid advanced by the same launches
+    (gkm_create_batch).  Each box has its own gas parameters and dt."""
+
+    def __init__(self, nx: int, gases, ny: int | None = None, nz: int | None = None, dx: float | None = None):
+        self.nx = int(nx)
+        self.ny = int(nx if ny is None else ny)
+        self.nz = int(nx if nz is None else nz)
+        self.dx = float(dx) if dx is not None else 2.0 * math.pi / self.nx
+        self.gases = list(gases)
+        self.nbox = len(self.gases)
+        self.shape = (self.ny + 6, self.nx + 6, self.nz + 6)
+        self.ncell = int(np.prod(self.shape))
+        self.dt = np.zeros(self.nbox)
+        self.t = np.zeros(self.nbox)
+        self.ics = [None] * self.nbox
+        self._L = _abi.load()
+        self._h = C.c_void_p()
+        grid = _abi.GkmGrid(self.nx, self.ny, self.nz, 3, self.dx)
+        arr = (_abi.GkmParams * self.nbox)(*[g.to_c() for g in self.gases])
+        _abi.check(self._L.gkm_create_batch(C.byref(grid), arr, self.nbox, C.byref(self._h)))
+
+    @classmethod
+    def from_ics(cls, ics, **gas_kw) -> "Batch":
+        """One box per DHIT initial condition (same grid), mu0 and T0 from each IC."""
+        R = gas_kw.pop("R", 287.0)
+        gases = [GasParams(gamma=ic.gamma, R=R, mu0=ic.mu0, T0=ic.p0 / (R * ic.rho0), **gas_kw) for ic in ics]
+        ny, nx, nz = ics[0].u.shape
+        b = cls(nx, gases, ny, nz)
+        for i, ic in enumerate(ics):
+            b.ics[i] = ic
+            b.set_state(i, conservative_padded(ic))
+        return b
+
+    def set_state(self, box: int, W) -> None:
+        W = [np.ascontiguousarray(np.asarray(w, dtype=np.float64).reshape(-1)) for w in W]
+        if len(W) != 5 or any(w.size != self.ncell for w in W):
+            raise ValueError(f"expected 5 planes of {self.ncell} values")
+        _abi.check(self._L.gkm_set_state_box(self._h, int(box), _abi.ptr_array(W)))
+
+    def get_state(self, box: int):
+        W = [np.empty(self.ncell) for _ in range(5)]
+        _abi.check(self._L.gkm_get_state_box(self._h, int(box), _abi.ptr_array(W)))
+        return W
+
+    def periodic_bc(self) -> None:
+        _abi.check(self._L.gkm_fill_ghosts(self._h))
+
+    def timestep(self, cfl: float) -> np.ndarray:
+        self.periodic_bc()
+        for b in range(self.nbox):
+            d = C.c_double()
+            _abi.check(self._L.gkm_compute_dt_box(self._h, b, float(cfl), C.byref(d)))
+            self.dt[b] = d.value
+        return self.dt.copy()
+
+    def step(self, nsteps: int = 1, dt=None) -> None:
+        dt = self.dt if dt is None else np.asarray(dt, dtype=np.float64)
+        dt = np.ascontiguousarray(np.broadcast_to(dt, (self.nbox,)), dtype=np.float64)
+        _abi.check(self._L.gkm_step_batch(self._h, dt.ctypes.data_as(_abi.DP), int(nsteps)))
+        self.t += nsteps * dt
+
+    def stats(self, box: int) -> dict:
+        s = _abi.GkmStats()
+        _abi.check(self._L.gkm_stats_box(self._h, int(box), C.byref(s)))
+        return s.as_dict()
+
+    def sync(self) -> None:
+        _abi.check(self._L.gkm_sync(self._h))
+
+    def set_graph(self, on: bool = True) -> None:
+        _abi.check(self._L.gkm_set_graph(self._h, int(on)))
+
+    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
+        _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.gkm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def flux_points(gas: GasParams, rec: np.ndarray, dx: float) -> np.ndarray:

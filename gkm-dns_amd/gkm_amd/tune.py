@@ -1,0 +1,93 @@
+"""Autotuner for the launch shapes of the step kernels.
+
+For a box size (and precision / reconstruction), every candidate pair
+(flux threads per block in {64, 128, 256}, WENO interfaces per thread of the
+strided sweeps in {4, 8, 16}) is timed on a batch of boxes of that size with
+device events around the step, after a warm-up step; the fastest pair is kept.
+Results are cached as JSON, keyed by "n/precision/recon/nbox", so later runs
+read them instead of timing again.  The arithmetic of every candidate is the
+same: the choice changes speed, never results.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .ic import dhit
+from .solver import Batch
+
+FLUX_BLOCKS = (64, 128, 256)
+RECON_CHUNKS = (4, 8, 16)
+DEFAULT = (256, 8)
+
+
+def cache_path() -> Path:
+    env = os.environ.get("GKM_TUNE_CACHE")
+    if env:
+        return Path(env)
+    return Path.home() / ".cache" / "gkm_amd" / "tuning.json"
+
+
+def _load(path: Path) -> dict:
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path: Path, data: dict) -> None:
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(json.dumps(data, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def tune_size(n: int, precision: str = "mixed", recon: str = "js", nbox: int = 4, steps: int = 3,
+              use_cache: bool = True, verbose: bool = False) -> tuple[int, int]:
+    key = f"{n}/{precision}/{recon}/{nbox}"
+    path = cache_path()
+    cache = _load(path) if use_cache else {}
+    if key in cache:
+        return tuple(cache[key]["best"])
+    ics = [dhit(n, Mt=0.3 + 0.05 * i, Re=70, seed=900 + i) for i in range(nbox)]
+    b = Batch.from_ics(ics, precision=precision, recon=recon, energy_recon="T", visc_law="power")
+    dt = b.timestep(0.5)
+    times = {}
+    for fb, rc in itertools.product(FLUX_BLOCKS, RECON_CHUNKS):
+        b.set_tuning(fb, rc)
+        b.step(1, dt)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps, dt)
+        b.sync()
+        times[(fb, rc)] = (time.perf_counter() - t0) / steps
+        if verbose:
+            print(f"tune n={n} flux_block={fb} recon_chunk={rc}: {1e3 * times[(fb, rc)]:.3f} ms/step")
+    b.close()
+    best = min(times, key=times.get)
+    cache[key] = {"best": list(best), "ms_per_step": {f"{k[0]}x{k[1]}": 1e3 * v for k, v in times.items()},
+                  "default_ms": 1e3 * times[DEFAULT]}
+    _store(path, cache)
+    return best
+
+
+def tune_sizes(sizes, **kw) -> dict[int, tuple[int, int]]:
+    return {int(n): tune_size(int(n), **kw) for n in sizes}
+
+
+def speedup_report(sizes, precision: str = "mixed", recon: str = "js", nbox: int = 4) -> dict:
+    cache = _load(cache_path())
+    rep = {}
+    for n in sizes:
+        e = cache.get(f"{n}/{precision}/{recon}/{nbox}")
+        if e:
+            best = min(e["ms_per_step"].values())
+            rep[int(n)] = {"best": e["best"], "best_ms": best, "default_ms": e["default_ms"],
+                           "gain": e["default_ms"] / best if best > 0 else float(np.nan)}
+    return rep

@@ -107,6 +107,31 @@ typedef struct gkm_ctx gkm_t;
 int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out);
 int gkm_destroy(gkm_t* h);
 
+/* A batch of nbox independent boxes of one grid, advanced by the same
+ * launches (one launch per stage covers every box).  params[b] gives box b's
+ * gas constants (gamma, R, Pr, mu0, T0, visc_exp, K, visc_law may differ);
+ * recon, energy_recon, zbug_compat and precision must agree.  No reference
+ * counterpart: the reference runs one segment per MPI rank.  The *_box calls
+ * address one box; the single-box calls above act on box 0. */
+int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, gkm_t** out);
+int gkm_batch_size(gkm_t* h, int* nbox);
+int gkm_set_state_box(gkm_t* h, int box, const double* const W[5]);
+int gkm_get_state_box(gkm_t* h, int box, double* const W[5]);
+/* dt[b] per box; nsteps steps of every box. */
+int gkm_step_batch(gkm_t* h, const double* dt, int nsteps);
+int gkm_compute_dt_box(gkm_t* h, int box, double cfl, double* dt);
+int gkm_stats_box(gkm_t* h, int box, gkm_stats_t* out);
+
+/* Capture one step (ghost fill, three sweeps, update) in a hipGraph on first
+ * use and replay it for every later step; recaptured when dt changes.  Off by
+ * default.  Steps timed with gkm_set_timing are launched without the graph. */
+int gkm_set_graph(gkm_t* h, int enabled);
+
+/* Launch shapes: threads per block of the flux kernels (64, 128 or 256) and
+ * interfaces per thread of the strided-sweep WENO kernels (4, 8 or 16).
+ * Chosen per box size by the autotuner in gkm_amd/tune.py. */
+int gkm_set_tuning(gkm_t* h, int flux_block, int recon_chunk);
+
 /* Copy the padded host state in / out (bit-exact round trip for FP64; the
  * mixed form splits each value into an FP32 hi/lo pair and rebuilds it).
  * Replaces Initialize_DHIT's writes into W and the H2D/D2H copies of
