@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-realization ensemble")
+    ap.add_argument("--no-slab", action="store_true", help="skip the 1024x1024x128 slab")
+    ap.add_argument("--slab-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=6)
@@ -147,6 +149,51 @@ def bench_ensemble(args):
             "tuning_gain": tune.speedup_report(sizes, precision="mixed", recon=args.recon)}
 
 
+def bench_slab(args):
+    """One rank's 1024 x 1024 x 128 slab of a 1024^3 box (Mt = 0.6, Re = 150), the
+    y faces exchanged by on-device pack / unpack (timed apart from the compute),
+    mixed precision; one FP64 step from the same state checks the increment."""
+    import numpy as np
+
+    import gkm_amd
+    from gkm_amd.ic import conservative_padded, synthetic_solenoidal
+
+    nx, ny, nz = 1024, 128, 1024
+    t0 = time.perf_counter()
+    ic = synthetic_solenoidal(nx, ny, nz, Mt=0.6, Re=150.0)
+    p0, mu0, rho0 = ic.p0, ic.mu0, ic.rho0
+    W0 = conservative_padded(ic)
+    del ic
+    print(f"[bench] slab initial state in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = {"box": [nx, ny, nz], "halo": "pack"}
+    out = {}
+    for prec in ("mixed", "fp64"):
+        gas = gkm_amd.GasParams(mu0=mu0, T0=p0 / (287.0 * rho0), visc_law="power", recon=args.recon,
+                                energy_recon="T", precision=prec)
+        s = gkm_amd.Solver(nx, ny, nz, gas=gas)
+        s.set_halo_mode("pack")
+        s.set_state(W0)
+        s.periodic_bc()
+        dt = out.get("dt") or s.timestep(args.cfl)
+        out["dt"] = dt
+        s.evolve_eulerian(1, dt=dt)
+        W1 = s.get_state()
+        sec, tim = time_solver(s, dt, args.slab_steps, 0)
+        s.close()
+        n = tim["steps"]
+        res[prec] = {"value": nx * ny * nz * args.slab_steps / sec, "ms_per_step": 1e3 * sec / args.slab_steps,
+                     "halo_ms_per_step": tim["ghost_ms"] / n, "prim_ms_per_step": tim["prim_ms"] / n,
+                     "compute_ms_per_step": (tim["flux_ms"] + tim["recon_ms"] + tim["update_ms"]) / n}
+        out[prec] = [w - w0 for w, w0 in zip(W1, W0)]  # one-step increment
+        print(f"[bench] slab {prec}: {res[prec]['ms_per_step']:.2f} ms/step, halo "
+              f"{res[prec]['halo_ms_per_step']:.3f} ms", file=sys.stderr, flush=True)
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(out["mixed"], out["fp64"]))
+    den = sum(float(np.sum(b ** 2)) for b in out["fp64"])
+    res["increment_rel_l2_mixed_vs_fp64"] = (num / den) ** 0.5
+    res["value"] = res["mixed"]["value"]
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -221,6 +268,8 @@ def main():
     }
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args)
+    if not args.no_slab:
+        out["slab"] = bench_slab(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, kw)
     print(json.dumps(out))

@@ -87,6 +87,8 @@ int upload_pars(Ctx* h, const double* dt) {
 }
 
 void free_ctx(Ctx* h) {
+    if (h->halo_send) (void)hipFree(h->halo_send);
+    if (h->halo_recv) (void)hipFree(h->halo_recv);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->arena) (void)hipFree(h->arena);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -292,7 +294,7 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         const bool timed = h->timing && s < gkm::kTimedSteps;
         if (timed) {
-            gkm::launch_step(h, h->ev_step + s * 12);
+            gkm::launch_step(h, h->ev_step + s * gkm::kEvPerStep);
             h->timed_steps = s + 1;
         } else if (h->graph_enabled) {
             if (!h->graph_ready) {
@@ -361,16 +363,18 @@ int gkm_set_timing(gkm_t* hh, int enabled) {
 int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
     Ctx* h = ctx(hh);
     if (!h || !ms) return fail(GKM_E_ARG, "null argument");
-    for (int i = 0; i < 4; i++) ms[i] = -1.0;
+    for (int i = 0; i < 5; i++) ms[i] = -1.0;
     if (nsteps) *nsteps = h->timed_steps;
     if (!h->timing || h->timed_steps == 0) return GKM_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    double flux = 0, recon = 0, ghost = 0, upd = 0;
+    double flux = 0, recon = 0, ghost = 0, upd = 0, prim = 0;
     for (int s = 0; s < h->timed_steps; s++) {
-        hipEvent_t* ev = h->ev_step + s * 12;
+        hipEvent_t* ev = h->ev_step + s * gkm::kEvPerStep;
         float t;
-        HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[12]));
         ghost += t;
+        HIP_TRY(hipEventElapsedTime(&t, ev[12], ev[1]));
+        prim += t;
         for (int d = 0; d < 3; d++) {
             HIP_TRY(hipEventElapsedTime(&t, ev[1 + 3 * d], ev[2 + 3 * d]));
             recon += t;
@@ -384,6 +388,24 @@ int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
     ms[1] = recon;
     ms[2] = ghost;
     ms[3] = upd;
+    ms[4] = prim;
+    return GKM_OK;
+}
+
+int gkm_set_halo_mode(gkm_t* hh, int mode) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (mode != GKM_HALO_GATHER && mode != GKM_HALO_PACK) return fail(GKM_E_ARG, "bad halo mode %d", mode);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (mode == GKM_HALO_PACK && !h->halo_send) {
+        // two 3-plane y faces of every plane (5 FP64 or 10 FP32) of every box
+        const size_t elem = h->par.precision == GKM_PREC_FP64 ? sizeof(double) : 2 * sizeof(float);
+        const size_t bytes = (size_t)2 * 3 * h->geo.s0 * 5 * elem * h->nbox;
+        HIP_TRY(hipMalloc(&h->halo_send, bytes));
+        HIP_TRY(hipMalloc(&h->halo_recv, bytes));
+    }
+    h->halo_mode = mode;
+    drop_graph(h);
     return GKM_OK;
 }
 

@@ -11,6 +11,9 @@
 namespace gkm {
 
 constexpr int kTimedSteps = 64;  // steps per gkm_step call that carry HIP events
+// events of one step: 0 start, 12 after the ghost fill, 1 after the primitives,
+// 1+3d / 2+3d / 3+3d around sweep d's WENO and flux launches, 11 after the update
+constexpr int kEvPerStep = 13;
 
 struct Ctx {
     gkm_grid grid;
@@ -52,10 +55,14 @@ struct Ctx {
     double* red_part = nullptr;  // partial sums per block
     double* red_out = nullptr;   // final values
     int red_blocks = 0;
+    // halo mode and the y-face exchange buffers of GKM_HALO_PACK
+    int halo_mode = GKM_HALO_GATHER;
+    void* halo_send = nullptr;
+    void* halo_recv = nullptr;
     // timing
     bool timing = false;
     int timed_steps = 0;
-    hipEvent_t ev_step[kTimedSteps * 12];
+    hipEvent_t ev_step[kTimedSteps * kEvPerStep];
     bool events_made = false;
 };
 

@@ -59,6 +59,70 @@ __global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Box3 all, Planes10<T>
 }
 
 // ---------------------------------------------------------------------------
+// Slab-exchange ghost fill (GKM_HALO_PACK).  The 3-plane y faces are
+// contiguous in this layout (y is the slowest axis): plane j starts at j*s0.
+// pack:   send[box][plane][side][e] <- face (side 0: j = 3..5, side 1: j = ny..ny+2)
+// (exchange: a ring neighbour's send buffer becomes this box's receive buffer;
+//  on one GPU with periodic ends it is a copy of its own)
+// unpack: ghosts j = 0..2 <- recv side 1, j = ny+3..ny+5 <- recv side 0
+// then the x and z ghosts of every plane, y ghosts included, take their
+// periodic image, which leaves edges and corners as the reference's y-then-x-
+// then-z exchange order does (update.cpp:894-1054).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_halo_pack(Geo g, Planes10<T> f, T* __restrict__ send, int npl) {
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned face = 3u * g.s0;
+    if (e >= face) return;
+    const unsigned q = blockIdx.y >> 1, side = blockIdx.y & 1u, box = blockIdx.z;
+    const unsigned src = box * g.ncell + (side ? (unsigned)g.ny : 3u) * g.s0 + e;
+    put(send, ((box * npl + q) * 2u + side) * face + e, ld(f.p[q], src));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_halo_unpack(Geo g, Planes10<T> f, const T* __restrict__ recv, int npl) {
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned face = 3u * g.s0;
+    if (e >= face) return;
+    const unsigned q = blockIdx.y >> 1, side = blockIdx.y & 1u, box = blockIdx.z;
+    // the low face (side 0) lands in the high ghosts, the high face in the low ghosts
+    const unsigned dst = box * g.ncell + (side ? 0u : (unsigned)(g.ny + 3)) * g.s0 + e;
+    put(f.p[q], dst, ld(recv, ((box * npl + q) * 2u + side) * face + e));
+}
+
+template <int NPL, class T>
+__global__ void __launch_bounds__(256) k_ghost_xz(Geo g, Box3 all, Planes10<T> f) {
+    const unsigned tl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tl >= all.count()) return;
+    const unsigned base = blockIdx.y * g.ncell;
+    const unsigned r = fdiv(tl, all.f2);
+    const int k = (int)(tl - r * (unsigned)all.n2);
+    const int j = (int)fdiv(r, all.f1);
+    const int i = (int)(r - (unsigned)j * (unsigned)all.n1);
+    const bool gi = i < 3 || i >= g.nx + 3, gk = k < 3 || k >= g.nz + 3;
+    if (!(gi || gk)) return;
+    const int is = i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
+    const int ks = k < 3 ? k + g.nz : (k >= g.nz + 3 ? k - g.nz : k);
+    const unsigned src = base + (unsigned)j * g.s0 + (unsigned)is * g.s1 + (unsigned)ks;
+#pragma unroll
+    for (int q = 0; q < NPL; q++) put(f.p[q], base + tl, ld(f.p[q], src));
+}
+
+template <class Real, class St>
+__global__ void __launch_bounds__(256) k_prim(Geo g, St st, Real* Tw, Real* tau, const StepPar* __restrict__ pars) {
+    const unsigned tl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tl >= g.ncell) return;
+    const unsigned t = blockIdx.y * g.ncell + tl;
+    const StepPar& par = pars[blockIdx.y];
+    const double r0 = st.get(0, t), m1 = st.get(1, t), m2 = st.get(2, t), m3 = st.get(3, t), E = st.get(4, t);
+    put(Tw, t, (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R));
+    const double u = m1 / r0, v = m2 / r0, w = m3 / r0;
+    const double P = (r0 * (par.gam - 1)) * (E / r0 - 0.5 * (u * u + v * v + w * w));
+    const double Ti = P / (par.R * r0);
+    put(tau, t, (Real)(viscosity(par, Ti) / P));
+}
+
+// ---------------------------------------------------------------------------
 // WENO reconstruction of one sweep on the extended interface set (normal
 // index [2, P-4], tangential [2, P-3]): each interface once.  The reference
 // evaluates the line of every transverse neighbour again inside each
@@ -446,12 +510,24 @@ void launch_ghost(Ctx* h, bool prim) {
     const Geo& g = h->geo;
     Box3 all = padded_box(g);
     const dim3 grid(blocks_for(all.count(), 256), h->nbox);
+    const bool pack = h->halo_mode == GKM_HALO_PACK;
     if (h->par.precision == GKM_PREC_FP64) {
         Planes10<double> f{};
         StateD st;
         for (int q = 0; q < 5; q++) f.p[q] = st.w[q] = h->Wd[q];
-        hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), grid, dim3(256), 0, h->stream, g, all, f, st,
-                           prim ? h->Td : nullptr, h->taud, h->d_pars);
+        if (pack) {
+            const unsigned face = 3u * g.s0;
+            const dim3 pg(blocks_for(face, 256), 2 * 5, h->nbox);
+            hipLaunchKernelGGL(k_halo_pack<double>, pg, dim3(256), 0, h->stream, g, f, (double*)h->halo_send, 5);
+            (void)hipMemcpyAsync(h->halo_recv, h->halo_send, sizeof(double) * 2 * face * 5 * h->nbox,
+                                 hipMemcpyDeviceToDevice, h->stream);
+            hipLaunchKernelGGL(k_halo_unpack<double>, pg, dim3(256), 0, h->stream, g, f, (const double*)h->halo_recv,
+                               5);
+            hipLaunchKernelGGL((k_ghost_xz<5, double>), grid, dim3(256), 0, h->stream, g, all, f);
+        } else {
+            hipLaunchKernelGGL((k_ghost_prim<5, double, double, StateD>), grid, dim3(256), 0, h->stream, g, all, f,
+                               st, prim ? h->Td : nullptr, h->taud, h->d_pars);
+        }
     } else {
         Planes10<float> f{};
         StateM st;
@@ -459,8 +535,37 @@ void launch_ghost(Ctx* h, bool prim) {
             f.p[q] = st.hi[q] = h->Whi[q];
             f.p[5 + q] = st.lo[q] = h->Wlo[q];
         }
-        hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), grid, dim3(256), 0, h->stream, g, all, f, st,
-                           prim ? h->Tf : nullptr, h->tauf, h->d_pars);
+        if (pack) {
+            const unsigned face = 3u * g.s0;
+            const dim3 pg(blocks_for(face, 256), 2 * 10, h->nbox);
+            hipLaunchKernelGGL(k_halo_pack<float>, pg, dim3(256), 0, h->stream, g, f, (float*)h->halo_send, 10);
+            (void)hipMemcpyAsync(h->halo_recv, h->halo_send, sizeof(float) * 2 * face * 10 * h->nbox,
+                                 hipMemcpyDeviceToDevice, h->stream);
+            hipLaunchKernelGGL(k_halo_unpack<float>, pg, dim3(256), 0, h->stream, g, f, (const float*)h->halo_recv,
+                               10);
+            hipLaunchKernelGGL((k_ghost_xz<10, float>), grid, dim3(256), 0, h->stream, g, all, f);
+        } else {
+            hipLaunchKernelGGL((k_ghost_prim<10, float, float, StateM>), grid, dim3(256), 0, h->stream, g, all, f, st,
+                               prim ? h->Tf : nullptr, h->tauf, h->d_pars);
+        }
+    }
+}
+
+// per-cell primitives when the ghost fill did not compute them (GKM_HALO_PACK)
+static void launch_prim(Ctx* h) {
+    const Geo& g = h->geo;
+    const dim3 grid(blocks_for(g.ncell, 256), h->nbox);
+    if (h->par.precision == GKM_PREC_FP64) {
+        StateD st;
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        hipLaunchKernelGGL((k_prim<double, StateD>), grid, dim3(256), 0, h->stream, g, st, h->Td, h->taud, h->d_pars);
+    } else {
+        StateM st;
+        for (int q = 0; q < 5; q++) {
+            st.hi[q] = h->Whi[q];
+            st.lo[q] = h->Wlo[q];
+        }
+        hipLaunchKernelGGL((k_prim<float, StateM>), grid, dim3(256), 0, h->stream, g, st, h->Tf, h->tauf, h->d_pars);
     }
 }
 
@@ -484,6 +589,8 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
     const Geo& g = h->geo;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     launch_ghost(h, true);
+    if (ev) (void)hipEventRecord(ev[12], h->stream);
+    if (h->halo_mode == GKM_HALO_PACK) launch_prim(h);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     Box3 bi = interior_box(g);
     const dim3 ugrid(blocks_for(bi.count(), 256), h->nbox);

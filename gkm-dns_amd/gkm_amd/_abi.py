@@ -31,8 +31,10 @@ EXPORTS = (
     "gkm_compute_dt", "gkm_stats", "gkm_sync", "gkm_set_timing", "gkm_last_timing", "gkm_flux_points",
     "gkm_weno_points", "gkm_last_error", "gkm_version",
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
-    "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning",
+    "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
 )
+
+HALO = {"gather": 0, "pack": 1}
 
 
 class GkmGrid(C.Structure):
@@ -104,6 +106,7 @@ def load() -> C.CDLL:
     L.gkm_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmStats)]
     L.gkm_set_graph.argtypes = [vp, C.c_int]
     L.gkm_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.gkm_set_halo_mode.argtypes = [vp, C.c_int]
     L.gkm_last_error.restype = C.c_char_p
     L.gkm_version.restype = C.c_char_p
     for name in EXPORTS:

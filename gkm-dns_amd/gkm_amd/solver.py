@@ -201,15 +201,21 @@ class Solver:
     def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
         _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
 
+    def set_halo_mode(self, mode: str = "gather") -> None:
+        """"gather" (periodic image per ghost cell) or "pack" (y-face pack /
+        exchange / unpack, then x/z ghosts; the slab-exchange path)."""
+        _abi.check(self._L.gkm_set_halo_mode(self._h, _abi.HALO[mode]))
+
     # -- timing -----------------------------------------------------------------------
     def set_timing(self, on: bool = True) -> None:
         _abi.check(self._L.gkm_set_timing(self._h, int(on)))
 
     def last_timing(self):
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * 5)()
         n = C.c_int()
         _abi.check(self._L.gkm_last_timing(self._h, ms, C.byref(n)))
-        return {"flux_ms": ms[0], "recon_ms": ms[1], "ghost_ms": ms[2], "update_ms": ms[3], "steps": n.value}
+        return {"flux_ms": ms[0], "recon_ms": ms[1], "ghost_ms": ms[2], "update_ms": ms[3], "prim_ms": ms[4],
+                "steps": n.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -304,6 +310,9 @@ class Batch:
 
     def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
         _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
+
+    def set_halo_mode(self, mode: str = "gather") -> None:
+        _abi.check(self._L.gkm_set_halo_mode(self._h, _abi.HALO[mode]))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -40,6 +40,12 @@ extern "C" {
 #define GKM_ENERGY_T 0         /* reconstruct T, rebuild rho*E (CPU path, update.cpp:560-619) */
 #define GKM_ENERGY_E 1         /* reconstruct rho*E directly (GPU path, GPUupdate.cu:546)     */
 
+#define GKM_HALO_GATHER 0      /* every ghost cell gathers its periodic image (fused with the
+                                  per-cell primitives)                                          */
+#define GKM_HALO_PACK   1      /* y faces packed into a send buffer and unpacked from a receive
+                                  buffer (the single-GPU stand-in for a slab exchange), then the
+                                  x/z ghosts filled in place                                      */
+
 #define GKM_PREC_FP64  0       /* FP64 everywhere (drop-in form)                               */
 #define GKM_PREC_MIXED 1       /* FP32 reconstruction + flux, FP32 hi/lo compensated state     */
 
@@ -164,10 +170,17 @@ int gkm_sync(gkm_t* h);
 /* Device time of the last gkm_step call, from HIP events recorded on the
  * handle's stream around each launch (first 64 steps of the call):
  * ms[0] BGK flux kernels, ms[1] WENO reconstruction kernels, ms[2] ghost
- * fills, ms[3] conservative updates; -1 where nothing was recorded.
+ * fill (in GKM_HALO_GATHER mode fused with the primitives), ms[3]
+ * conservative updates, ms[4] the separate primitives kernel of
+ * GKM_HALO_PACK mode (0 otherwise); -1 where nothing was recorded.
  * *nsteps receives the number of timed steps.  Timing is off by default. */
 int gkm_set_timing(gkm_t* h, int enabled);
-int gkm_last_timing(gkm_t* h, double ms[4], int* nsteps);
+int gkm_last_timing(gkm_t* h, double ms[5], int* nsteps);
+
+/* Ghost-fill mode (GKM_HALO_*).  The y-face pack/unpack path is the one a
+ * slab decomposition exchanges between ranks (replaces the pack, Sendrecv and
+ * unpack of HaloTransfer, weno/src/update.cpp:894-1054, y faces :899-936). */
+int gkm_set_halo_mode(gkm_t* h, int mode);
 
 /* Point-wise entry points used to check the device arithmetic against the
  * reference on sampled interfaces.  Replace flux() (GKMflux.cpp:11-215) and

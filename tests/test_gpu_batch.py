@@ -89,3 +89,21 @@ def test_small_ensemble_mixed_vs_fp64(gk):
         assert abs(a["mt"] - b["mt"]) <= 1e-3 * abs(b["mt"])
     em.close()
     ed.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pack_halo_matches_gather(gk, prec):
+    """y-face pack / exchange / unpack + x/z fill gives the same ghosts, hence
+    the same steps, as the per-cell periodic gather (uneven batch of 2)."""
+    ics = [gk.dhit(10, ny=7, nz=12, Mt=0.3 + 0.1 * i, seed=70 + i) for i in range(2)]
+    a = gk.Batch.from_ics(ics, precision=prec)
+    b = gk.Batch.from_ics(ics, precision=prec)
+    b.set_halo_mode("pack")
+    dts = a.timestep(0.5)
+    a.step(4, dts)
+    b.step(4, dts)
+    for i in range(2):
+        for x, y in zip(a.get_state(i), b.get_state(i)):
+            assert np.array_equal(x, y)
+    a.close()
+    b.close()
