@@ -35,6 +35,20 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_VEC_TFLOPS = 78.6         # MI355X vector FP64
 FP32_VEC_TFLOPS = 157.3        # MI355X vector FP32
 ALG_BYTES_PER_CELL = 80.0      # 5 state words read + 5 written per cell-update, 8 B each
+# measured once with rocprofv3 on the 256^3 workload (see the files for the commands)
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_flux_fp32_256_pmc.md"
+FLOPS_PROFILE = ROOT / "profiles" / "k_flux_256_valu_flops.md"
+
+
+def profile_number(path, key):
+    """A `key: value` line of a committed rocprofv3 summary, or None."""
+    try:
+        for line in Path(path).read_text().splitlines():
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def parse():
@@ -237,6 +251,20 @@ def main():
     alg_bytes_launch = ALG_BYTES_PER_CELL * cells / 3.0
     achieved = alg_bytes_launch / (flux_ms * 1e-3) / 1e9
     step_hbm = ALG_BYTES_PER_CELL * rate_m / 1e9
+    traffic = profile_number(TRAFFIC_PROFILE, "traffic_bytes_per_launch") if n == 256 else None
+    flops32 = profile_number(FLOPS_PROFILE, "flops_per_launch_fp32") if n == 256 else None
+    flops64 = profile_number(FLOPS_PROFILE, "flops_per_launch_fp64") if n == 256 else None
+    valu = None
+    if flops32:
+        tf = flops32 / (flux_ms * 1e-3) / 1e12
+        valu = {"bound": "valu", "achieved": tf, "peak": FP32_VEC_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP32_VEC_TFLOPS, "flops_per_launch": flops32,
+                "note": "FP32 vector peak includes packed FP32; this build compiles with packed FP32 off, "
+                        "so the reachable non-packed rate is half of it"}
+        if res_fp64 and flops64:
+            tf64 = flops64 / (res_fp64["flux_kernel_ms"] * 1e-3) / 1e12
+            valu["fp64"] = {"achieved": tf64, "peak": FP64_VEC_TFLOPS, "frac": tf64 / FP64_VEC_TFLOPS,
+                            "flops_per_launch": flops64}
 
     out = {
         "metric": "cell-updates/s, mixed vs FP64, 256^3 on 1 MI355X; % HBM roofline; E_k err",
@@ -255,10 +283,11 @@ def main():
                                f"WENO-{args.recon.upper()} + BGK, CFL {args.cfl}, mixed precision",
                    "box": [n, n, n], "ghost": 3, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_flux (BGK flux sweep)", "kernel_ms": flux_ms,
                      "alg_bytes_per_launch": alg_bytes_launch,
                      "step_frac": step_hbm / HBM_PEAK_GBS},
+        "roofline_valu": valu,
         "breakdown_ms_per_step": {k: v / tim_m["steps"] for k, v in tim_m.items() if k != "steps"},
         "fp64": res_fp64,
         "mixed_over_fp64": (rate_m / res_fp64["value"]) if res_fp64 else None,
