@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--slab-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     return ap.parse_args()
 
 
@@ -104,6 +104,8 @@ def cpu_baseline(args, gas_kw):
     O.step(g, P, W, dt, args.cpu_steps, 1)
     t1 = time.perf_counter()
     one = n ** 3 * args.cpu_steps / (t1 - t0)
+    print(f"[bench] cpu oracle, 1 thread: {t1 - t0:.1f} s for {args.cpu_steps} steps of {n}^3", file=sys.stderr,
+          flush=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
     t0 = time.perf_counter()
