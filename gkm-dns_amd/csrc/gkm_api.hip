@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "gkm_internal.h"
 
@@ -89,6 +90,8 @@ int upload_pars(Ctx* h, const double* dt) {
 void free_ctx(Ctx* h) {
     if (h->halo_send) (void)hipFree(h->halo_send);
     if (h->halo_recv) (void)hipFree(h->halo_recv);
+    if (h->spl_grad) (void)hipFree(h->spl_grad);
+    if (h->spl_inv) (void)hipFree(h->spl_inv);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->arena) (void)hipFree(h->arena);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -480,6 +483,86 @@ int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
 }
 
 int gkm_stats(gkm_t* h, gkm_stats_t* out) { return gkm_stats_box(h, 0, out); }
+
+int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!out) return fail(GKM_E_ARG, "null argument");
+    const gkm::Geo& g = h->geo;
+    if (!h->spl_grad) {
+        // Thomas pivots of the natural-spline slope system for each line length
+        std::vector<double> inv;
+        for (int m : {g.P0, g.P1, g.P2}) {
+            double prev = 0.0;
+            for (int i = 0; i < m; i++) {
+                const double b = (i == 0 || i == m - 1) ? 2.0 : 4.0;
+                prev = 1.0 / (i == 0 ? b : b - prev);
+                inv.push_back(prev);
+            }
+        }
+        hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
+        if (e != hipSuccess) {
+            h->spl_grad = nullptr;
+            return fail(GKM_E_NOMEM, "gradient planes (%zu bytes): %s", sizeof(double) * 9 * (size_t)h->ncell,
+                        hipGetErrorString(e));
+        }
+        HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
+        HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    }
+    gkm::launch_ghost(h, false);
+    double a[gkm::kPStat];
+    HIP_TRY(gkm::launch_print_stats(h, box, a));
+    const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
+    gkm_print_stats_t s;
+    std::memset(&s, 0, sizeof s);
+    s.tke = a[0] / N;
+    s.diss = a[1] / N;
+    s.sijsij = a[2] / N;
+    const double u2 = a[3] / N, u3 = a[4] / N, u4 = a[5] / N;
+    const double v2 = a[6] / N, v3 = a[7] / N, v4 = a[8] / N;
+    const double w2 = a[9] / N, w3 = a[10] / N, w4 = a[11] / N;
+    s.skew_u = u3 / std::pow(u2, 1.5);
+    s.skew_v = v3 / std::pow(v2, 1.5);
+    s.skew_w = w3 / std::pow(w2, 1.5);
+    s.kurt_u = u4 / (u2 * u2);
+    s.kurt_v = v4 / (v2 * v2);
+    s.kurt_w = w4 / (w2 * w2);
+    s.dudx2 = u2;
+    s.dvdy2 = v2;
+    s.dwdz2 = w2;
+    s.p_mean = a[12] / N;
+    s.T_mean = a[13] / N;
+    s.v_mean = a[14] / N;
+    // second pass: squared deviations of (rho, p, T, 1/rho) about the means
+    const double m[4] = {0.0, s.p_mean, s.T_mean, s.v_mean};
+    double b[gkm::kStatB];
+    HIP_TRY(gkm::launch_stats_b(h, box, m, b));
+    s.p_rms = std::sqrt(b[1] / N);
+    s.T_rms = std::sqrt(b[2] / N);
+    s.v_rms = std::sqrt(b[3] / N);
+    *out = s;
+    if (!(s.p_mean > 0) || !(s.v_mean > 0) || !std::isfinite(s.tke))
+        return fail(GKM_E_NONPHYSICAL, "non-physical state: <p>=%g <1/rho>=%g", s.p_mean, s.v_mean);
+    return GKM_OK;
+}
+
+int gkm_print_stats(gkm_t* h, gkm_print_stats_t* out) { return gkm_print_stats_box(h, 0, out); }
+
+int gkm_format_print_stats(const gkm_print_stats_t* s, double t_over_t0, double gamma, double p0, double T0,
+                           double Mt, double den0, char* buf, int len) {
+    if (!s || !buf || len <= 0) return fail(GKM_E_ARG, "null argument or empty buffer");
+    const double prms = s->p_rms / (gamma * p0 * Mt * Mt);
+    const double trms = s->T_rms / ((gamma - 1) * T0 * Mt * Mt);
+    const double vrms = s->v_rms / (Mt * Mt / den0);
+    // column separators as update.cpp:779-780 writes them (two spaces, four
+    // before Vrms)
+    const int w = snprintf(buf, (size_t)len, "%.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e    %.8e",
+                           t_over_t0, s->tke, s->diss, s->skew_u, s->skew_v, s->skew_w, s->kurt_u, s->kurt_v,
+                           s->kurt_w, prms, trms, vrms);
+    if (w < 0 || w >= len) return fail(GKM_E_ARG, "buffer of %d bytes too small for the %d-byte line", len, w);
+    return GKM_OK;
+}
 
 static int points_common(const gkm_params* p, int n, const double* in, size_t in_per, double* out, size_t out_per,
                          double** din, double** dout, hipStream_t* s) {

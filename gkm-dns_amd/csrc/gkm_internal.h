@@ -55,6 +55,10 @@ struct Ctx {
     double* red_part = nullptr;  // partial sums per block
     double* red_out = nullptr;   // final values
     int red_blocks = 0;
+    // PrintStats: nine FP64 gradient planes of one box and the Thomas pivots
+    // of the three line lengths (P0, P1, P2 values), allocated on first use
+    double* spl_grad = nullptr;
+    double* spl_inv = nullptr;
     // halo mode and the y-face exchange buffers of GKM_HALO_PACK
     int halo_mode = GKM_HALO_GATHER;
     void* halo_send = nullptr;
@@ -81,5 +85,8 @@ constexpr int kStatA = 28;  // sums + one max (slot 27)
 constexpr int kStatB = 4;
 hipError_t launch_stats_a(Ctx* h, int box, double* host_out /*kStatA*/);
 hipError_t launch_stats_b(Ctx* h, int box, const double means[4], double* host_out /*kStatB*/);
+constexpr int kPStat = 15;  // PrintStats sums
+// spline slopes into h->spl_grad (allocated by the caller), then the sums
+hipError_t launch_print_stats(Ctx* h, int box, double* host_out /*kPStat*/);
 
 }  // namespace gkm

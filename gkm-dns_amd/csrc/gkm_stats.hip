@@ -238,4 +238,173 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
     return hipStreamSynchronize(h->stream);
 }
 
+// ---------------------------------------------------------------------------
+// PrintStats with spline gradients (weno/src/update.cpp:638-784).
+//
+// k_spline<A>: one thread per grid line along storage axis A (0 = y, 1 = x,
+// 2 = z) through interior nodes.  It walks the whole padded line once forward
+// and once back, solving for the slopes of the natural cubic spline through
+// u, v, w = (rho u, rho v, rho w)/rho at its m nodes:
+//   2 D_0 + D_1 = (3/h)(f_1 - f_0)
+//   D_{i-1} + 4 D_i + D_{i+1} = (3/h)(f_{i+1} - f_{i-1})
+//   D_{m-2} + 2 D_{m-1} = (3/h)(f_{m-1} - f_{m-2})
+// The matrix is the same for every line of the axis, so the Thomas pivots
+// inv_i = 1/(b_i - inv_{i-1}) come precomputed: forward y_i = (r_i - y_{i-1})
+// inv_i, backward D_i = y_i - inv_i D_{i+1}.  The forward values of nodes
+// 3..m-1 go to the gradient plane and are overwritten by the slopes on the
+// way back; only interior nodes' slopes are used.
+// grad planes: (3 A + comp) * ncell.
+// ---------------------------------------------------------------------------
+template <class St, int A>
+__global__ void __launch_bounds__(256) k_spline(Geo g, St st, const double* __restrict__ inv, double r3h,
+                                                double* __restrict__ grad) {
+    const int m = A == 0 ? g.P0 : (A == 1 ? g.P1 : g.P2);
+    // lines: the two other axes over their interior ranges, the faster one inner
+    const int n_in = A == 2 ? g.nx : g.nz;
+    const int n_out = A == 0 ? g.nx : g.ny;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)n_in * (unsigned)n_out) return;
+    const unsigned a_in = t % (unsigned)n_in + 3, a_out = t / (unsigned)n_in + 3;
+    unsigned base, stride;
+    if (A == 0) {  // y lines over (x, z)
+        base = a_out * g.s1 + a_in;
+        stride = g.s0;
+    } else if (A == 1) {  // x lines over (y, z)
+        base = a_out * g.s0 + a_in;
+        stride = g.s1;
+    } else {  // z lines over (y, x)
+        base = a_out * g.s0 + a_in * g.s1;
+        stride = 1;
+    }
+    double* G0 = grad + (size_t)(3 * A) * g.ncell;
+    double* G1 = G0 + g.ncell;
+    double* G2 = G1 + g.ncell;
+    auto velo = [&](unsigned c, double f[3]) {
+        const double r = st.get(0, c);
+        f[0] = st.get(1, c) / r;
+        f[1] = st.get(2, c) / r;
+        f[2] = st.get(3, c) / r;
+    };
+    double fm[3], f0[3], fp[3], y[3] = {0.0, 0.0, 0.0};
+    velo(base, f0);
+    velo(base + stride, fp);
+    fm[0] = f0[0]; fm[1] = f0[1]; fm[2] = f0[2];
+    for (int i = 0; i < m; i++) {
+        const unsigned c = base + (unsigned)i * stride;
+        const double w = inv[i];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            // one-sided differences at the two ends, central inside
+            const double hi = i == m - 1 ? f0[q] : fp[q];
+            const double lo = i == 0 ? f0[q] : fm[q];
+            y[q] = (r3h * (hi - lo) - y[q]) * w;
+        }
+        if (i >= 3) {
+            put(G0, c, y[0]);
+            put(G1, c, y[1]);
+            put(G2, c, y[2]);
+        }
+        if (i + 1 < m) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                fm[q] = f0[q];
+                f0[q] = fp[q];
+            }
+            if (i + 2 < m) velo(c + 2 * stride, fp);
+        }
+    }
+    // y now holds y_{m-1} = D_{m-1}; back-substitute down to node 3
+    double d[3] = {y[0], y[1], y[2]};
+    for (int i = m - 2; i >= 3; i--) {
+        const unsigned c = base + (unsigned)i * stride;
+        const double w = inv[i];
+        d[0] = ld(G0, c) - w * d[0];
+        d[1] = ld(G1, c) - w * d[1];
+        d[2] = ld(G2, c) - w * d[2];
+        put(G0, c, d[0]);
+        put(G1, c, d[1]);
+        put(G2, c, d[2]);
+    }
+}
+
+// PrintStats sums over the interior (update.cpp:658-707):
+//  0 rho|u|^2  1 mu du_i/dx_j du_i/dx_j  2 S_ij S_ij  3..5 (du/dx)^2,3,4
+//  6..8 (dv/dy)^n  9..11 (dw/dz)^n  12 p  13 T  14 1/rho
+template <class St>
+__global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const double* __restrict__ grad, StatPar p,
+                                                        double* part) {
+    double acc[kPStat];
+#pragma unroll
+    for (int q = 0; q < kPStat; q++) acc[q] = 0.0;
+    const long n = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long r = t / g.nz;
+        const int i = (int)(r % g.nx);
+        const int j = (int)(r / g.nx);
+        const unsigned c = (unsigned)(j + 3) * g.s0 + (unsigned)(i + 3) * g.s1 + (unsigned)(k + 3);
+        const double rho = st.get(0, c);
+        double u[3];
+        vel(st, c, u);
+        const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
+        const double T = P / (p.R * rho);
+        auto gr = [&](int a, int comp) { return ld(grad + (size_t)(3 * a + comp) * g.ncell, c); };
+        const double dudx = gr(1, 0), dudy = gr(0, 0), dudz = gr(2, 0);
+        const double dvdx = gr(1, 1), dvdy = gr(0, 1), dvdz = gr(2, 1);
+        const double dwdx = gr(1, 2), dwdy = gr(0, 2), dwdz = gr(2, 2);
+        const double mu = visc(p, T);
+        acc[0] += rho * q2;
+        acc[1] += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                        dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
+        acc[2] += dudx * dudx + 0.5 * (dudy + dvdx) * (dudy + dvdx) + 0.5 * (dudz + dwdx) * (dudz + dwdx) +
+                  0.5 * (dwdy + dvdz) * (dwdy + dvdz) + dvdy * dvdy + dwdz * dwdz;
+        double x2 = dudx * dudx; acc[3] += x2; acc[4] += x2 * dudx; acc[5] += x2 * x2;
+        x2 = dvdy * dvdy; acc[6] += x2; acc[7] += x2 * dvdy; acc[8] += x2 * x2;
+        x2 = dwdz * dwdz; acc[9] += x2; acc[10] += x2 * dwdz; acc[11] += x2 * x2;
+        acc[12] += P; acc[13] += T; acc[14] += 1.0 / rho;
+    }
+    block_reduce_store<kPStat, 0u>(acc, part);
+}
+
+template <class St>
+static void launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
+    const Geo& g = h->geo;
+    const double r3h = 3.0 / h->grid.dx;
+    const double* inv0 = h->spl_inv;
+    const double* inv1 = inv0 + g.P0;
+    const double* inv2 = inv1 + g.P1;
+    const unsigned ly = (unsigned)g.nx * g.nz, lx = (unsigned)g.ny * g.nz, lz = (unsigned)g.ny * g.nx;
+    hipLaunchKernelGGL((k_spline<St, 0>), dim3((ly + 255) / 256), dim3(256), 0, h->stream, g, st, inv0, r3h,
+                       h->spl_grad);
+    hipLaunchKernelGGL((k_spline<St, 1>), dim3((lx + 255) / 256), dim3(256), 0, h->stream, g, st, inv1, r3h,
+                       h->spl_grad);
+    hipLaunchKernelGGL((k_spline<St, 2>), dim3((lz + 255) / 256), dim3(256), 0, h->stream, g, st, inv2, r3h,
+                       h->spl_grad);
+    hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, h->stream, g, st, h->spl_grad, p,
+                       h->red_part);
+}
+
+hipError_t launch_print_stats(Ctx* h, int box, double* host_out) {
+    const StatPar p = stat_par(h, box);
+    const size_t off = (size_t)box * h->ncell;
+    if (h->par.precision == GKM_PREC_FP64) {
+        StateD st;
+        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
+        launch_print_stats_st(h, st, p);
+    } else {
+        StateM st;
+        for (int q = 0; q < 5; q++) {
+            st.hi[q] = h->Whi[q] + off;
+            st.lo[q] = h->Wlo[q] + off;
+        }
+        launch_print_stats_st(h, st, p);
+    }
+    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, h->red_blocks,
+                       h->red_part, h->red_out);
+    hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kPStat, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(h->stream);
+}
+
 }  // namespace gkm

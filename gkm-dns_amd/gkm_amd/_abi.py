@@ -32,6 +32,7 @@ EXPORTS = (
     "gkm_weno_points", "gkm_last_error", "gkm_version",
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
+    "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats",
 )
 
 HALO = {"gather": 0, "pack": 1}
@@ -64,6 +65,19 @@ class GkmStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f in STAT_FIELDS}
+
+
+PRINT_STAT_FIELDS = (
+    "tke", "diss", "sijsij", "skew_u", "skew_v", "skew_w", "kurt_u", "kurt_v", "kurt_w",
+    "p_mean", "T_mean", "v_mean", "p_rms", "T_rms", "v_rms", "dudx2", "dvdy2", "dwdz2",
+)
+
+
+class GkmPrintStats(C.Structure):
+    _fields_ = [(f, C.c_double) for f in PRINT_STAT_FIELDS]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f in PRINT_STAT_FIELDS}
 
 
 class GkmError(RuntimeError):
@@ -107,6 +121,9 @@ def load() -> C.CDLL:
     L.gkm_set_graph.argtypes = [vp, C.c_int]
     L.gkm_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.gkm_set_halo_mode.argtypes = [vp, C.c_int]
+    L.gkm_print_stats.argtypes = [vp, C.POINTER(GkmPrintStats)]
+    L.gkm_print_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmPrintStats)]
+    L.gkm_format_print_stats.argtypes = [C.POINTER(GkmPrintStats)] + [C.c_double] * 6 + [C.c_char_p, C.c_int]
     L.gkm_last_error.restype = C.c_char_p
     L.gkm_version.restype = C.c_char_p
     for name in EXPORTS:
@@ -119,6 +136,17 @@ def load() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != GKM_OK:
         raise GkmError(rc, load().gkm_last_error().decode(errors="replace"))
+
+
+def format_print_stats(s: dict, t_over_t0: float, gamma: float, p0: float, T0: float, Mt: float,
+                       den0: float) -> str:
+    """PrintStats' output line (weno/src/update.cpp:779-780) through
+    gkm_format_print_stats."""
+    st = GkmPrintStats(**{f: float(s[f]) for f in PRINT_STAT_FIELDS})
+    buf = C.create_string_buffer(512)
+    check(load().gkm_format_print_stats(C.byref(st), float(t_over_t0), float(gamma), float(p0), float(T0), float(Mt),
+                                        float(den0), buf, len(buf)))
+    return buf.value.decode()
 
 
 def ptr_array(arrays) -> C.Array:

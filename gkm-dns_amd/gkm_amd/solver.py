@@ -152,25 +152,23 @@ class Solver:
         return s.as_dict()
 
     def print_stats(self, Mt: float | None = None, p0: float | None = None, T0: float | None = None,
-                    rho0: float = 1.0, file=None) -> dict:
-        """Statistics with PrintStats' columns (update.cpp:748-781):
-        t/t0 TKE Diss SKEWu SKEWv SKEWw KURTu KURTv KURTw Prms Trms Vrms.
-        Gradients are 2nd-order central differences."""
-        s = self.stats()
+                    rho0: float | None = None, file=None) -> dict:
+        """PrintStats (update.cpp:638-784): natural-spline velocity gradients
+        over the padded box, the reference's sums, and its output line
+        t/t0 TKE Diss SKEWu SKEWv SKEWw KURTu KURTv KURTw Prms Trms Vrms
+        (gkm_print_stats + gkm_format_print_stats).  Mt, p0, den0 default to
+        the initial condition's (the reference's params globals)."""
+        st = _abi.GkmPrintStats()
+        _abi.check(self._L.gkm_print_stats(self._h, C.byref(st)))
+        s = st.as_dict()
         ic = getattr(self, "ic", None)
-        Mt = Mt if Mt is not None else (ic.Mt if ic else s["mt"])
+        Mt = Mt if Mt is not None else (ic.Mt if ic else 1.0)
         p0 = p0 if p0 is not None else (ic.p0 if ic else s["p_mean"])
+        rho0 = rho0 if rho0 is not None else (ic.rho0 if ic else 1.0)
         T0 = T0 if T0 is not None else self.gas.T0
-        g = self.gas.gamma
-        prms = s["p_rms"] / (g * p0 * Mt * Mt)
-        trms = s["T_rms"] / ((g - 1) * T0 * Mt * Mt)
-        vrms = s["v_rms"] / (Mt * Mt / rho0)
-        cols = [self.info.t / self.info.t0, s["tke"], s["diss_ref"], s["skew_u"], s["skew_v"], s["skew_w"],
-                s["kurt_u"], s["kurt_v"], s["kurt_w"], prms, trms, vrms]
-        line = "  ".join(f"{x:.8e}" for x in cols)
+        s["line"] = _abi.format_print_stats(s, self.info.t / self.info.t0, self.gas.gamma, p0, T0, Mt, rho0)
         if file is not None:
-            print(line, file=file)
-        s["line"] = line
+            print(s["line"], file=file)
         return s
 
     def evolve(self, t_end: float, cfl: float = 0.02, stats_every: int = 1, recompute_dt: bool = False,
@@ -300,6 +298,12 @@ class Batch:
     def stats(self, box: int) -> dict:
         s = _abi.GkmStats()
         _abi.check(self._L.gkm_stats_box(self._h, int(box), C.byref(s)))
+        return s.as_dict()
+
+    def print_stats(self, box: int) -> dict:
+        """PrintStats quantities of one box (gkm_print_stats_box)."""
+        s = _abi.GkmPrintStats()
+        _abi.check(self._L.gkm_print_stats_box(self._h, int(box), C.byref(s)))
         return s.as_dict()
 
     def sync(self) -> None:

@@ -105,6 +105,28 @@ typedef struct gkm_stats_s {
     double mass, mom_x, mom_y, mom_z, energy; /* interior sums of W0..W4       */
 } gkm_stats_t;
 
+/* The quantities PrintStats writes (weno/src/update.cpp:638-784), FP64.
+ * Velocity gradients are the slopes at the grid nodes of the natural cubic
+ * spline through the padded box's values of u = rho*u/rho (and v, w) along
+ * each axis.  The reference evaluates the gradient of a tensor-product cubic
+ * B-spline with natural ends built over the padded segment
+ * (get_bsplinep, weno/src/Interpolation.cpp:3-31; generateSplines,
+ * update.cpp:1215-1231); at grid nodes that gradient is exactly these 1-D
+ * slopes.  Means are over the nx*ny*nz interior cells; mu follows the
+ * handle's viscosity law (PrintStats uses Sutherland, update.cpp:698).
+ * rms values are unnormalised; gkm_format_print_stats applies the
+ * reference's scalings. */
+typedef struct gkm_print_stats_s {
+    double tke;                 /* <rho u_i u_i>                   (TKE)      */
+    double diss;                /* <mu du_i/dx_j du_i/dx_j>        (Diss)     */
+    double sijsij;              /* <S_ij S_ij>                     (SijSij)   */
+    double skew_u, skew_v, skew_w;   /* <(du/dx)^3>/<(du/dx)^2>^1.5 ...    */
+    double kurt_u, kurt_v, kurt_w;   /* <(du/dx)^4>/<(du/dx)^2>^2 ...      */
+    double p_mean, T_mean, v_mean;   /* v = 1/rho                          */
+    double p_rms, T_rms, v_rms;      /* rms about the means                */
+    double dudx2, dvdy2, dwdz2;      /* <(du/dx)^2>, <(dv/dy)^2>, <(dw/dz)^2> */
+} gkm_print_stats_t;
+
 typedef struct gkm_ctx gkm_t;
 
 /* Create a handle on the current HIP device and allocate the device state.
@@ -163,6 +185,21 @@ int gkm_compute_dt(gkm_t* h, double cfl, double* dt);
 /* FP64 device reductions of the interior (replaces the sums of PrintStats,
  * update.cpp:638-784, and the GKM on-device TKE/PRMS, GPUupdate.cu:425-483). */
 int gkm_stats(gkm_t* h, gkm_stats_t* out);
+
+/* PrintStats (weno/src/update.cpp:638-784) with its spline gradients
+ * (generateSplines + eval_UBspline_3d_d_vg, update.cpp:1215-1231, :679-681):
+ * ghost fill, natural-spline slopes of u, v, w along every grid line of the
+ * padded box, then FP64 interior reductions.  The first call allocates nine
+ * FP64 gradient planes of one box (freed by gkm_destroy). */
+int gkm_print_stats(gkm_t* h, gkm_print_stats_t* out);
+int gkm_print_stats_box(gkm_t* h, int box, gkm_print_stats_t* out);
+/* PrintStats' stdout line (update.cpp:778-781), "%.8e" columns:
+ *   t/t0  TKE  Diss  SKEWu  SKEWv  SKEWw  KURTu  KURTv  KURTw
+ *   Prms/(gamma p0 Mt^2)  Trms/((gamma-1) T0 Mt^2)  Vrms/(Mt^2/den0)
+ * with the reference's separators; no newline.  Returns GKM_E_ARG if the
+ * line does not fit in len bytes. */
+int gkm_format_print_stats(const gkm_print_stats_t* s, double t_over_t0, double gamma, double p0, double T0,
+                           double Mt, double den0, char* buf, int len);
 
 /* Wait for all work queued on the handle's stream. */
 int gkm_sync(gkm_t* h);

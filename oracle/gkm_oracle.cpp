@@ -714,4 +714,129 @@ void orc_stats(const gkm_grid* g, const gkm_params* p, double* const Win[5], gkm
     *out = s;
 }
 
+
+/* Natural cubic spline slopes by the Thomas algorithm on the slope form of the
+ * continuity conditions (second derivative zero at both ends):
+ *   2 D_0 + D_1 = 3 (f_1 - f_0)/h
+ *   D_{i-1} + 4 D_i + D_{i+1} = 3 (f_{i+1} - f_{i-1})/h,  0 < i < m-1
+ *   D_{m-2} + 2 D_{m-1} = 3 (f_{m-1} - f_{m-2})/h
+ * At grid nodes the gradient of the tensor-product interpolating B-spline with
+ * natural ends that einspline builds per axis (get_bsplinep,
+ * weno/src/Interpolation.cpp:3-31) equals these 1-D slopes along the line. */
+void orc_spline_slopes(const double* f, long stride, int m, double h, double* D) {
+    std::vector<double> cp((size_t)m), y((size_t)m);
+    for (int i = 0; i < m; i++) {
+        double r;
+        if (i == 0) r = 3.0 * (f[stride] - f[0]) / h;
+        else if (i == m - 1) r = 3.0 * (f[(long)(m - 1) * stride] - f[(long)(m - 2) * stride]) / h;
+        else r = 3.0 * (f[(long)(i + 1) * stride] - f[(long)(i - 1) * stride]) / h;
+        const double b = (i == 0 || i == m - 1) ? 2.0 : 4.0;
+        const double den = i == 0 ? b : b - cp[i - 1];
+        cp[i] = 1.0 / den;
+        y[i] = i == 0 ? r / den : (r - y[i - 1]) / den;
+    }
+    D[m - 1] = y[m - 1];
+    for (int i = m - 2; i >= 0; i--) D[i] = y[i] - cp[i] * D[i + 1];
+}
+
+/* PrintStats (weno/src/update.cpp:638-784): first pass means of p, T, 1/rho
+ * and the gradient sums (:658-707), second pass rms about the means
+ * (:751-763); velocities u = W1/rho etc. over the padded segment
+ * (generateSplines, :1215-1231); gradients from the natural spline. */
+void orc_print_stats(const gkm_grid* g, const gkm_params* p, double* const Win[5], gkm_print_stats_t* out) {
+    Box B(*g);
+    std::vector<double> store((size_t)B.n * 5);
+    double* W[5];
+    for (int q = 0; q < 5; q++) {
+        W[q] = &store[(size_t)q * B.n];
+        std::memcpy(W[q], Win[q], sizeof(double) * B.n);
+    }
+    fill_ghosts(B, W);
+    const double gam = p->gamma, R = p->R;
+    const double N = (double)g->nx * g->ny * g->nz;
+    /* velocity fields of the whole padded box */
+    std::vector<double> vel((size_t)B.n * 3);
+    for (long c = 0; c < B.n; c++) {
+        const double den = W[0][c];
+        vel[c] = W[1][c] / den;
+        vel[B.n + c] = W[2][c] / den;
+        vel[2 * B.n + c] = W[3][c] / den;
+    }
+    /* grad[(3 * a + comp) * n + c]: slope of velocity comp along storage axis a
+     * (0 = y, 1 = x, 2 = z), on lines through interior nodes */
+    std::vector<double> grad((size_t)B.n * 9, 0.0);
+    std::vector<double> line;
+    for (int a = 0; a < 3; a++) {
+        const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+        const int m = B.P[a];
+        line.resize((size_t)m);
+        for (int u1 = 3; u1 < B.P[b1] - 3; u1++)
+            for (int u2 = 3; u2 < B.P[b2] - 3; u2++) {
+                const long base = u1 * B.s[b1] + u2 * B.s[b2];
+                for (int comp = 0; comp < 3; comp++) {
+                    const double* f = &vel[(size_t)comp * B.n + base];
+                    orc_spline_slopes(f, B.s[a], m, B.dx, line.data());
+                    double* G = &grad[(size_t)(3 * a + comp) * B.n + base];
+                    for (int t = 0; t < m; t++) G[t * B.s[a]] = line[t];
+                }
+            }
+    }
+    auto gr = [&](int a, int comp, long c) { return grad[(size_t)(3 * a + comp) * B.n + c]; };
+    double sTKE = 0, sDiss = 0, sS = 0, sP = 0, sT = 0, sV = 0;
+    double u2 = 0, u3 = 0, u4 = 0, v2 = 0, v3 = 0, v4 = 0, w2 = 0, w3 = 0, w4 = 0;
+    for (int j = 3; j < B.P[0] - 3; j++)
+        for (int i = 3; i < B.P[1] - 3; i++)
+            for (int k = 3; k < B.P[2] - 3; k++) {
+                const long c = B.at(j, i, k);
+                const double deni = W[0][c];
+                const double ui = W[1][c] / deni, vi = W[2][c] / deni, wi = W[3][c] / deni;
+                const double Vsqr = (ui * ui + vi * vi + wi * wi);
+                const double Pi = (deni * (gam - 1)) * (W[4][c] / deni - 0.5 * Vsqr);
+                const double Ti = Pi / (R * deni);
+                sP += Pi;
+                sT += Ti;
+                sV += 1 / deni;
+                const double dudx = gr(1, 0, c), dudy = gr(0, 0, c), dudz = gr(2, 0, c);
+                const double dvdx = gr(1, 1, c), dvdy = gr(0, 1, c), dvdz = gr(2, 1, c);
+                const double dwdx = gr(1, 2, c), dwdy = gr(0, 2, c), dwdz = gr(2, 2, c);
+                const double mu = viscosity(*p, Ti);
+                sDiss += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                               dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
+                sTKE += deni * (ui * ui + vi * vi + wi * wi);
+                sS += (dudx * dudx + 0.5 * std::pow(dudy + dvdx, 2) + 0.5 * std::pow(dudz + dwdx, 2) +
+                       0.5 * std::pow(dwdy + dvdz, 2) + dvdy * dvdy + dwdz * dwdz);
+                u2 += dudx * dudx; u3 += dudx * dudx * dudx; u4 += dudx * dudx * dudx * dudx;
+                v2 += dvdy * dvdy; v3 += dvdy * dvdy * dvdy; v4 += dvdy * dvdy * dvdy * dvdy;
+                w2 += dwdz * dwdz; w3 += dwdz * dwdz * dwdz; w4 += dwdz * dwdz * dwdz * dwdz;
+            }
+    gkm_print_stats_t s;
+    std::memset(&s, 0, sizeof s);
+    s.tke = sTKE / N;
+    s.diss = sDiss / N;
+    s.sijsij = sS / N;
+    u2 /= N; u3 /= N; u4 /= N; v2 /= N; v3 /= N; v4 /= N; w2 /= N; w3 /= N; w4 /= N;
+    s.skew_u = u3 / std::pow(u2, 1.5); s.skew_v = v3 / std::pow(v2, 1.5); s.skew_w = w3 / std::pow(w2, 1.5);
+    s.kurt_u = u4 / std::pow(u2, 2); s.kurt_v = v4 / std::pow(v2, 2); s.kurt_w = w4 / std::pow(w2, 2);
+    s.dudx2 = u2; s.dvdy2 = v2; s.dwdz2 = w2;
+    s.p_mean = sP / N; s.T_mean = sT / N; s.v_mean = sV / N;
+    double rp = 0, rT = 0, rV = 0;
+    for (int j = 3; j < B.P[0] - 3; j++)
+        for (int i = 3; i < B.P[1] - 3; i++)
+            for (int k = 3; k < B.P[2] - 3; k++) {
+                const long c = B.at(j, i, k);
+                const double deni = W[0][c];
+                const double ui = W[1][c] / deni, vi = W[2][c] / deni, wi = W[3][c] / deni;
+                const double Vsqr = (ui * ui + vi * vi + wi * wi);
+                const double Pi = (deni * (gam - 1)) * (W[4][c] / deni - 0.5 * Vsqr);
+                const double Ti = Pi / (R * deni);
+                rp += std::pow((s.p_mean - Pi), 2);
+                rT += std::pow((s.T_mean - Ti), 2);
+                rV += std::pow((s.v_mean - 1 / deni), 2);
+            }
+    s.p_rms = std::sqrt(rp / N);
+    s.T_rms = std::sqrt(rT / N);
+    s.v_rms = std::sqrt(rV / N);
+    *out = s;
+}
+
 }  // extern "C"

@@ -55,6 +55,13 @@ void orc_fluxes(const gkm_grid* g, const gkm_params* p, double* const W[5], doub
 double orc_timestep(const gkm_grid* g, const gkm_params* p, double* const W[5], double cfl);
 void orc_stats(const gkm_grid* g, const gkm_params* p, double* const W[5], gkm_stats_t* out);
 
+/* Slopes D[0..m-1] at the nodes of the natural cubic spline through the m >= 2
+ * values f[0], f[stride], ..., f[(m-1)*stride] spaced h apart. */
+void orc_spline_slopes(const double* f, long stride, int m, double h, double* D);
+/* PrintStats (weno/src/update.cpp:638-784) with natural-spline gradients over
+ * the padded box (after a ghost fill of a copy of W). */
+void orc_print_stats(const gkm_grid* g, const gkm_params* p, double* const W[5], gkm_print_stats_t* out);
+
 #ifdef __cplusplus
 }
 #endif

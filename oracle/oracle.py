@@ -49,6 +49,19 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f in STAT_FIELDS}
 
 
+PRINT_STAT_FIELDS = [
+    "tke", "diss", "sijsij", "skew_u", "skew_v", "skew_w", "kurt_u", "kurt_v", "kurt_w",
+    "p_mean", "T_mean", "v_mean", "p_rms", "T_rms", "v_rms", "dudx2", "dvdy2", "dwdz2",
+]
+
+
+class PrintStats(C.Structure):
+    _fields_ = [(f, C.c_double) for f in PRINT_STAT_FIELDS]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in PRINT_STAT_FIELDS}
+
+
 _lib = None
 _ref = None
 
@@ -71,6 +84,8 @@ def lib():
         L.orc_timestep.argtypes = [C.POINTER(Grid), C.POINTER(Params), C.POINTER(DP), C.c_double]
         L.orc_timestep.restype = C.c_double
         L.orc_stats.argtypes = [C.POINTER(Grid), C.POINTER(Params), C.POINTER(DP), C.POINTER(Stats)]
+        L.orc_spline_slopes.argtypes = [DP, C.c_long, C.c_int, C.c_double, DP]
+        L.orc_print_stats.argtypes = [C.POINTER(Grid), C.POINTER(Params), C.POINTER(DP), C.POINTER(PrintStats)]
         _lib = L
     return _lib
 
@@ -191,6 +206,20 @@ def timestep(grid: Grid, params: Params, W, cfl: float) -> float:
 def stats(grid: Grid, params: Params, W) -> dict:
     s = Stats()
     lib().orc_stats(C.byref(grid), C.byref(params), _ptrs(W), C.byref(s))
+    return s.as_dict()
+
+
+def spline_slopes(f, h: float) -> np.ndarray:
+    """Natural cubic spline slopes at the nodes of a uniformly spaced line."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    D = np.zeros(f.size)
+    lib().orc_spline_slopes(_p(f), 1, int(f.size), float(h), _p(D))
+    return D
+
+
+def print_stats(grid: Grid, params: Params, W) -> dict:
+    s = PrintStats()
+    lib().orc_print_stats(C.byref(grid), C.byref(params), _ptrs(W), C.byref(s))
     return s.as_dict()
 
 

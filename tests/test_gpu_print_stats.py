@@ -1,0 +1,125 @@
+"""PrintStats on the device (gkm_print_stats: natural-spline gradients over
+the padded box + FP64 reductions) against the CPU oracle's orc_print_stats,
+which tests/test_print_stats.py pins against scipy's natural cubic spline.
+
+Bar: 1e-11 relative per quantity for the FP64 state; the mixed form is
+compared on its own state (read back as hi + lo), to 1e-10.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from _util import smooth_box
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gk():
+    import gkm_amd
+    gkm_amd.load()
+    return gkm_amd
+
+
+def _gas(gk, **kw):
+    base = dict(gamma=1.4, R=287.0, Pr=0.7, mu0=3e-3, T0=1.0 / (1.4 * 287.0), visc_law="sutherland",
+                recon="symoo", energy_recon="T", zbug_compat=True, precision="fp64")
+    base.update(kw)
+    return gk.GasParams(**base)
+
+
+def _orc_params(orc, gas):
+    import gkm_amd._abi as A
+    return orc.make_params(gamma=gas.gamma, R=gas.R, Pr=gas.Pr, mu0=gas.mu0, T0=gas.T0, visc_exp=gas.visc_exp,
+                           K=gas.dof(), visc_law=A.VISC[gas.visc_law], recon=A.RECON[gas.recon],
+                           energy_recon=A.ENERGY[gas.energy_recon], zbug_compat=int(gas.zbug_compat), precision=0)
+
+
+def _rough_state(orc, nx, ny, nz, seed):
+    """Smooth field plus cell noise, so skewness and kurtosis are far from
+    their symmetric values; ghosts filled."""
+    rng = np.random.default_rng(seed)
+    grid = orc.make_grid(nx, ny, nz, 2 * math.pi / nx)
+    W = smooth_box(nx, ny, nz, seed=seed, mach=0.4)
+    orc.fill_ghosts(grid, W)
+    W[0] = W[0] * (1 + 0.2 * rng.random(W[0].size))
+    for q in (1, 2, 3):
+        W[q] = W[q] + 0.3 * rng.normal(size=W[q].size)
+    W[4] = W[4] + 0.5 * (W[1] ** 2 + W[2] ** 2 + W[3] ** 2) / W[0]
+    orc.fill_ghosts(grid, W)
+    return grid, W
+
+
+def _check(dev, ref, tol):
+    for k, v in ref.items():
+        assert abs(dev[k] - v) <= tol * abs(v) + 1e-300, (k, dev[k], v)
+
+
+@pytest.mark.parametrize("shape", [(12, 10, 14), (7, 9, 5), (40, 33, 36)])
+@pytest.mark.parametrize("law", ["sutherland", "power"])
+def test_print_stats_fp64(gk, orc, shape, law):
+    nx, ny, nz = shape
+    print("print_stats fp64", shape, law, flush=True)
+    grid, W = _rough_state(orc, nx, ny, nz, seed=nx + 3 * ny)
+    gas = _gas(gk, visc_law=law)
+    with gk.Solver(nx, ny, nz, gas=gas) as s:
+        s.set_state(W)
+        dev = s.print_stats()
+    ref = orc.print_stats(grid, _orc_params(orc, gas), W)
+    _check(dev, ref, 1e-11)
+
+
+def test_print_stats_mixed(gk, orc):
+    nx, ny, nz = 16, 12, 20
+    print("print_stats mixed", (nx, ny, nz), flush=True)
+    grid, W = _rough_state(orc, nx, ny, nz, seed=5)
+    gas = _gas(gk, precision="mixed")
+    with gk.Solver(nx, ny, nz, gas=gas) as s:
+        s.set_state(W)
+        Wm = s.get_state()
+        dev = s.print_stats()
+    ref = orc.print_stats(grid, _orc_params(orc, gas), Wm)
+    _check(dev, ref, 1e-10)
+
+
+def test_print_stats_after_steps_and_line(gk, orc):
+    """DHIT 24^3 after 5 FP64 steps: device vs oracle on the evolved state, and
+    the formatted line carries the same numbers."""
+    print("print_stats dhit 24^3", flush=True)
+    s = gk.Solver.from_dhit(24, k0=4, Mt=0.4, Re=50, seed=8, visc_law="sutherland", recon="js")
+    dt = s.timestep(0.5)
+    s.evolve_eulerian(5, dt=dt)
+    d = s.print_stats()
+    W = s.get_state()
+    grid = orc.make_grid(24, 24, 24, 2 * math.pi / 24)
+    ref = orc.print_stats(grid, _orc_params(orc, s.gas), W)
+    _check({k: d[k] for k in ref}, ref, 1e-11)
+    cols = [float(x) for x in d["line"].split()]
+    ic = s.ic
+    g = s.gas.gamma
+    want = [s.info.t / s.info.t0, ref["tke"], ref["diss"], ref["skew_u"], ref["skew_v"], ref["skew_w"],
+            ref["kurt_u"], ref["kurt_v"], ref["kurt_w"], ref["p_rms"] / (g * ic.p0 * ic.Mt ** 2),
+            ref["T_rms"] / ((g - 1) * s.gas.T0 * ic.Mt ** 2), ref["v_rms"] / (ic.Mt ** 2 / ic.rho0)]
+    assert len(cols) == 12
+    for a, b in zip(cols, want):
+        assert abs(a - b) <= 1e-8 * abs(b), (a, b)
+    s.close()
+
+
+def test_print_stats_batch_equals_single(gk, orc):
+    """Box b of a batch gives bit-for-bit the single-box result."""
+    nx, ny, nz = 10, 12, 9
+    print("print_stats batch", flush=True)
+    states = [_rough_state(orc, nx, ny, nz, seed=40 + b)[1] for b in range(3)]
+    gases = [_gas(gk, mu0=1e-3 * (b + 1)) for b in range(3)]
+    b = gk.Batch(nx, gases, ny=ny, nz=nz)
+    for i, W in enumerate(states):
+        b.set_state(i, W)
+    got = [b.print_stats(i) for i in range(3)]
+    b.close()
+    for i, W in enumerate(states):
+        with gk.Solver(nx, ny, nz, gas=gases[i]) as s:
+            s.set_state(W)
+            d = s.print_stats()
+            assert {k: d[k] for k in got[i]} == got[i]
