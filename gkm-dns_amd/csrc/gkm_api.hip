@@ -100,6 +100,9 @@ void free_ctx(Ctx* h) {
 
 }  // namespace
 
+// error reporting for the other host translation units (gkm_io.cpp)
+int gkm::report_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 extern "C" {
 
 const char* gkm_last_error(void) { return g_err.c_str(); }

@@ -70,6 +70,9 @@ struct Ctx {
     bool events_made = false;
 };
 
+// sets the message gkm_last_error() returns on this thread; returns code
+int report_error(int code, const char* msg);
+
 StepPar step_par(const gkm_grid& grid, const gkm_params& p, double dt);
 void launch_ghost(Ctx* h, bool prim);  // prim = false: ghosts only
 void launch_step(Ctx* h, hipEvent_t* ev);  // ev: 12 events or nullptr

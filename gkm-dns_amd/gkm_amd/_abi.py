@@ -17,6 +17,7 @@ GKM_E_HIP = -2
 GKM_E_NONPHYSICAL = -3
 GKM_E_NOMEM = -4
 GKM_E_UNSUPPORTED = -5
+GKM_E_IO = -6
 
 VISC = {"sutherland": 0, "power": 1}
 RECON = {"symoo": 0, "js": 1}
@@ -33,6 +34,9 @@ EXPORTS = (
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
     "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats",
+    # include/gkm_io.h
+    "gkm_read_weno_params", "gkm_write_weno_params", "gkm_read_weno_init", "gkm_write_weno_init",
+    "gkm_read_init_dat", "gkm_write_init_dat", "gkm_init_dat_state",
 )
 
 HALO = {"gather": 0, "pack": 1}
@@ -80,6 +84,16 @@ class GkmPrintStats(C.Structure):
         return {f: getattr(self, f) for f in PRINT_STAT_FIELDS}
 
 
+class GkmWenoParams(C.Structure):
+    _fields_ = [("nc", C.c_int), ("gam", C.c_double), ("den0", C.c_double), ("Mt", C.c_double), ("Re", C.c_double),
+                ("mu0", C.c_double), ("p0", C.c_double), ("t0", C.c_double)]
+
+
+class GkmInitDat(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(f, C.c_float) for f in
+                                   ("rho0", "Mt", "Re", "k0", "urms", "mu", "crms", "p0", "eddy_time")]
+
+
 class GkmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"gkm error {code}: {msg}")
@@ -124,6 +138,15 @@ def load() -> C.CDLL:
     L.gkm_print_stats.argtypes = [vp, C.POINTER(GkmPrintStats)]
     L.gkm_print_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmPrintStats)]
     L.gkm_format_print_stats.argtypes = [C.POINTER(GkmPrintStats)] + [C.c_double] * 6 + [C.c_char_p, C.c_int]
+    FP = C.POINTER(C.c_float)
+    IP = C.POINTER(C.c_int)
+    L.gkm_read_weno_params.argtypes = [C.c_char_p, C.POINTER(GkmWenoParams)]
+    L.gkm_write_weno_params.argtypes = [C.c_char_p, C.POINTER(GkmWenoParams)]
+    L.gkm_read_weno_init.argtypes = [C.c_char_p, C.c_int, IP, C.c_double, C.POINTER(DP)]
+    L.gkm_write_weno_init.argtypes = [C.c_char_p, C.c_int, IP, C.c_double, C.POINTER(DP)]
+    L.gkm_read_init_dat.argtypes = [C.c_char_p, C.POINTER(GkmInitDat), FP, FP, FP]
+    L.gkm_write_init_dat.argtypes = [C.c_char_p, C.POINTER(GkmInitDat), FP, FP, FP]
+    L.gkm_init_dat_state.argtypes = [C.POINTER(GkmInitDat), FP, FP, FP, C.c_double, C.POINTER(DP)]
     L.gkm_last_error.restype = C.c_char_p
     L.gkm_version.restype = C.c_char_p
     for name in EXPORTS:

@@ -5,6 +5,8 @@ The reference drives one rank's segment through free functions and globals
 
     GetParams()            -> Solver(...) / Solver.from_dhit(...)
     Initialize_DHIT(W)     -> Solver.initialize_dhit(ic) / Solver.set_state(W)
+    GetParams + Initialize_DHIT from a case folder -> Solver.from_weno_case(dir)
+    WriteReInit(W)         -> Solver.write_reinit(dir)
     PeriodicBC(W)          -> Solver.periodic_bc()
     timestep(W)            -> Solver.timestep(cfl)            (sets Solver.dt)
     evolveEulerian(W, dev) -> Solver.evolve_eulerian(nsteps)  (one step each)
@@ -109,8 +111,49 @@ class Solver:
         """Initialize_DHIT (FlowField.cpp:59-112) followed by the first halo fill."""
         self.ic = ic
         self.info.t0 = ic.t0
+        self.norm = {"Mt": ic.Mt, "p0": ic.p0, "rho0": ic.rho0}
         self.set_state(conservative_padded(ic))
         self.periodic_bc()
+
+    @classmethod
+    def from_weno_case(cls, case_dir, proc_dim=(1, 1, 1), precision: str = "fp64", visc_law: str = "sutherland",
+                       recon: str = "symoo", energy_recon: str = "T", zbug_compat: bool = True, Pr: float = 0.7,
+                       R: float = 287.0, visc_exp: float = 0.76) -> "Solver":
+        """GetParams + Initialize_DHIT from a reference case folder (params and
+        the Init%i%i%i rank files of a proc_dim decomposition); defaults are
+        the weno code's (Sutherland, SYMOO, T reconstruction, Pr 0.7)."""
+        from pathlib import Path
+
+        from . import io
+        p = io.read_weno_params(Path(case_dir) / "params")
+        gas = GasParams(gamma=p.gam, R=R, Pr=Pr, mu0=p.mu0, T0=p.T0(R), visc_law=visc_law, visc_exp=visc_exp,
+                        recon=recon, energy_recon=energy_recon, zbug_compat=zbug_compat, precision=precision,
+                        K=p.K())
+        s = cls(p.nc, p.nc, p.nc, dx=p.dx(), gas=gas)
+        s.set_state(io.read_weno_init(case_dir, p.nc, p.gam, proc_dim))
+        s.periodic_bc()
+        s.info.t0 = p.t0
+        s.norm = {"Mt": p.Mt, "p0": p.p0, "rho0": p.den0}
+        s.case = p
+        return s
+
+    def write_reinit(self, case_dir, proc_dim=(1, 1, 1)) -> None:
+        """WriteReInit (update.cpp:1352-1391): Init%i%i%i rank files of the
+        current state (a cubic box), readable by from_weno_case."""
+        from . import io
+        if not self.nx == self.ny == self.nz:
+            raise ValueError("the rank-file format describes a cubic box")
+        io.write_weno_init(case_dir, self.get_state(), self.nx, self.gas.gamma, proc_dim)
+
+    def save_checkpoint(self, path) -> None:
+        """Exact checkpoint (.npz): state, grid, gas parameters, clock."""
+        from . import io
+        io.save_checkpoint(path, self)
+
+    @classmethod
+    def load_checkpoint(cls, path) -> "Solver":
+        from . import io
+        return io.load_checkpoint(path)
 
     # -- state ------------------------------------------------------------------------
     def set_state(self, W) -> None:
@@ -157,14 +200,14 @@ class Solver:
         over the padded box, the reference's sums, and its output line
         t/t0 TKE Diss SKEWu SKEWv SKEWw KURTu KURTv KURTw Prms Trms Vrms
         (gkm_print_stats + gkm_format_print_stats).  Mt, p0, den0 default to
-        the initial condition's (the reference's params globals)."""
+        the initial condition's or case's (the reference's params globals)."""
         st = _abi.GkmPrintStats()
         _abi.check(self._L.gkm_print_stats(self._h, C.byref(st)))
         s = st.as_dict()
-        ic = getattr(self, "ic", None)
-        Mt = Mt if Mt is not None else (ic.Mt if ic else 1.0)
-        p0 = p0 if p0 is not None else (ic.p0 if ic else s["p_mean"])
-        rho0 = rho0 if rho0 is not None else (ic.rho0 if ic else 1.0)
+        norm = getattr(self, "norm", None) or {}
+        Mt = Mt if Mt is not None else norm.get("Mt", 1.0)
+        p0 = p0 if p0 is not None else norm.get("p0", s["p_mean"])
+        rho0 = rho0 if rho0 is not None else norm.get("rho0", 1.0)
         T0 = T0 if T0 is not None else self.gas.T0
         s["line"] = _abi.format_print_stats(s, self.info.t / self.info.t0, self.gas.gamma, p0, T0, Mt, rho0)
         if file is not None:

@@ -1,0 +1,103 @@
+"""Restart and the command-line driver on the device.
+
+* gkm_dns (the C++ driver of a reference case folder) prints the same
+  PrintStats lines as the Python host mirror stepping the same case, and
+  writes the same restart files;
+* a run restarted from WriteReInit rank files matches the uninterrupted run
+  to 1e-13 relative L2 (the files hold primitives, so the restart state
+  differs from the live one by roundings, as in the reference);
+* the exact checkpoint restarts the FP64 form bit for bit.
+"""
+import subprocess
+
+import numpy as np
+import pytest
+
+from _util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gk():
+    import gkm_amd
+    gkm_amd.load()
+    return gkm_amd
+
+
+def test_driver_matches_host_mirror(gk, tmp_path):
+    from gkm_amd import io
+    from gkm_amd._abi import LIB_PATH
+    from gkm_amd.ic import dhit
+    ic = dhit(16, Mt=0.4, Re=40, seed=77)
+    case = tmp_path / "case"
+    io.write_weno_case(case, ic, proc_dim=(2, 2, 1))
+    print("gkm_dns 16^3 x 4 steps", flush=True)
+    r = subprocess.run([str(LIB_PATH.parent / "gkm_dns"), str(case), "--proc-dim", "2", "2", "1", "--steps", "4",
+                        "--cfl", "0.3", "--reinit", str(tmp_path / "out_drv")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [ln for ln in r.stdout.splitlines() if ln.count("e") >= 12 and ln[:1].isdigit()]
+    assert len(lines) == 5
+    s = gk.Solver.from_weno_case(case, proc_dim=(2, 2, 1))
+    s.timestep(0.3)
+    mine = []
+    for _ in range(4):
+        mine.append(s.print_stats()["line"])
+        s.evolve_eulerian(1)
+    mine.append(s.print_stats()["line"])
+    assert mine == lines
+    s.write_reinit(tmp_path / "out_py", proc_dim=(2, 2, 1))
+    s.close()
+    for name in ("Init000", "Init010", "Init100", "Init110"):
+        assert (tmp_path / "out_drv" / name).read_bytes() == (tmp_path / "out_py" / name).read_bytes(), name
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_reinit_restart(gk, tmp_path, prec):
+    print("reinit restart", prec, flush=True)
+    kw = dict(k0=4, Mt=0.4, Re=50, seed=5, visc_law="sutherland", precision=prec)
+    a = gk.Solver.from_dhit(20, **kw)
+    dt = a.timestep(0.5)
+    a.evolve_eulerian(6, dt=dt)
+    Wa = a.get_state()
+    a.close()
+    b = gk.Solver.from_dhit(20, **kw)
+    b.evolve_eulerian(3, dt=dt)
+    b.write_reinit(tmp_path / "re")
+    gas = b.gas
+    b.close()
+    # WriteReInit writes the rank files only; the gas comes from the first run
+    c = gk.Solver(20, gas=gas)
+    from gkm_amd import io
+    c.set_state(io.read_weno_init(tmp_path / "re", 20, gas.gamma))
+    c.periodic_bc()
+    c.evolve_eulerian(3, dt=dt)
+    Wc = c.get_state()
+    c.close()
+    assert rel_l2(Wc, Wa) <= 1e-13
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_exact_checkpoint(gk, tmp_path, prec):
+    print("checkpoint", prec, flush=True)
+    kw = dict(k0=4, Mt=0.4, Re=50, seed=6, visc_law="power", recon="js", precision=prec)
+    a = gk.Solver.from_dhit(18, **kw)
+    dt = a.timestep(0.5)
+    a.evolve_eulerian(3, dt=dt)
+    a.save_checkpoint(tmp_path / "ck.npz")
+    a.evolve_eulerian(3, dt=dt)
+    Wa = a.get_state()
+    la = a.print_stats()["line"]
+    a.close()
+    b = gk.Solver.load_checkpoint(tmp_path / "ck.npz")
+    assert b.info.steps == 3 and b.info.dt == dt
+    b.evolve_eulerian(3)
+    Wb = b.get_state()
+    lb = b.print_stats()["line"]
+    b.close()
+    if prec == "fp64":
+        assert all(np.array_equal(x, y) for x, y in zip(Wa, Wb))
+        assert la == lb
+    else:
+        assert rel_l2(Wb, Wa) <= 1e-13
