@@ -327,6 +327,115 @@ __global__ void __launch_bounds__(256) k_spline(Geo g, St st, const double* __re
     }
 }
 
+// z lines run along the unit stride, so one thread per line would touch a
+// different cache line on every lane.  k_spline_z gives a 64-lane block 64
+// lines and walks them in chunks of 16 nodes staged through LDS: the block
+// loads and stores 16-node row segments (128 B of doubles per line), and
+// each lane runs the same Thomas recursion as k_spline on its own line out
+// of LDS.
+constexpr int kSzLines = 64, kSzChunk = 16;
+
+template <class St>
+__global__ void __launch_bounds__(kSzLines) k_spline_z(Geo g, St st, const double* __restrict__ inv, double r3h,
+                                                       double* __restrict__ grad) {
+    __shared__ double tile[3][kSzLines][kSzChunk + 1];
+    const int lane = threadIdx.x;
+    const int m = g.P2;
+    const int nline = g.ny * g.nx;
+    const int line0 = blockIdx.x * kSzLines;
+    double* G[3] = {grad + (size_t)6 * g.ncell, grad + (size_t)7 * g.ncell, grad + (size_t)8 * g.ncell};
+    auto line_base = [&](int id) -> unsigned {
+        const unsigned j = (unsigned)(id / g.nx) + 3, i = (unsigned)(id % g.nx) + 3;
+        return j * g.s0 + i * g.s1;
+    };
+    const bool mine = line0 + lane < nline;
+    const unsigned base = mine ? line_base(line0 + lane) : 0u;
+    // cooperative pass over the block's lines x chunk nodes [n0, n0 + kSzChunk):
+    // element e = r * 64 + lane covers line e / kSzChunk, node e % kSzChunk
+    auto coop = [&](int n0, auto&& body) {
+#pragma unroll 4
+        for (int r = 0; r < kSzLines * kSzChunk / kSzLines; r++) {
+            const int e = r * kSzLines + lane;
+            const int l = e / kSzChunk, t = e % kSzChunk;
+            const int n = n0 + t;
+            if (line0 + l < nline && n < m) body(l, t, line_base(line0 + l) + (unsigned)n);
+        }
+    };
+    auto velo = [&](unsigned c, double f[3]) {
+        const double r = st.get(0, c);
+        f[0] = st.get(1, c) / r;
+        f[1] = st.get(2, c) / r;
+        f[2] = st.get(3, c) / r;
+    };
+    double f0[3] = {0.0, 0.0, 0.0}, fm[3], y[3] = {0.0, 0.0, 0.0};
+    if (mine) velo(base, f0);
+    fm[0] = f0[0]; fm[1] = f0[1]; fm[2] = f0[2];
+    const int nch = (m + kSzChunk - 1) / kSzChunk;
+    // forward: the tile holds f at nodes n0+1 .. n0+16; lane l walks nodes
+    // n0 .. n0+15 of its line and leaves y_n where it read f_{n+1}
+    for (int ch = 0; ch < nch; ch++) {
+        const int n0 = ch * kSzChunk;
+        coop(n0 + 1, [&](int l, int t, unsigned c) {
+            double f[3];
+            velo(c, f);
+            tile[0][l][t] = f[0];
+            tile[1][l][t] = f[1];
+            tile[2][l][t] = f[2];
+        });
+        __syncthreads();
+        for (int t = 0; t < kSzChunk; t++) {
+            const int n = n0 + t;
+            if (n >= m) break;
+            const double w = inv[n];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const double fp = tile[q][lane][t];  // node n+1 (not used at n = m-1)
+                const double hi = n == m - 1 ? f0[q] : fp;
+                const double lo = n == 0 ? f0[q] : fm[q];
+                y[q] = (r3h * (hi - lo) - y[q]) * w;
+                tile[q][lane][t] = y[q];
+                fm[q] = f0[q];
+                f0[q] = fp;
+            }
+        }
+        __syncthreads();
+        coop(n0, [&](int l, int t, unsigned c) {
+            put(G[0], c, tile[0][l][t]);
+            put(G[1], c, tile[1][l][t]);
+            put(G[2], c, tile[2][l][t]);
+        });
+        __syncthreads();
+    }
+    // backward from D_{m-1} = y_{m-1}: D_n = y_n - inv_n D_{n+1}
+    double d[3] = {y[0], y[1], y[2]};
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int n0 = ch * kSzChunk;
+        coop(n0, [&](int l, int t, unsigned c) {
+            tile[0][l][t] = ld(G[0], c);
+            tile[1][l][t] = ld(G[1], c);
+            tile[2][l][t] = ld(G[2], c);
+        });
+        __syncthreads();
+        for (int t = kSzChunk - 1; t >= 0; t--) {
+            const int n = n0 + t;
+            if (n >= m - 1) continue;  // node m-1 keeps y_{m-1}
+            const double w = inv[n];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                d[q] = tile[q][lane][t] - w * d[q];
+                tile[q][lane][t] = d[q];
+            }
+        }
+        __syncthreads();
+        coop(n0, [&](int l, int t, unsigned c) {
+            put(G[0], c, tile[0][l][t]);
+            put(G[1], c, tile[1][l][t]);
+            put(G[2], c, tile[2][l][t]);
+        });
+        __syncthreads();
+    }
+}
+
 // PrintStats sums over the interior (update.cpp:658-707):
 //  0 rho|u|^2  1 mu du_i/dx_j du_i/dx_j  2 S_ij S_ij  3..5 (du/dx)^2,3,4
 //  6..8 (dv/dy)^n  9..11 (dw/dz)^n  12 p  13 T  14 1/rho
@@ -379,8 +488,8 @@ static void launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
                        h->spl_grad);
     hipLaunchKernelGGL((k_spline<St, 1>), dim3((lx + 255) / 256), dim3(256), 0, h->stream, g, st, inv1, r3h,
                        h->spl_grad);
-    hipLaunchKernelGGL((k_spline<St, 2>), dim3((lz + 255) / 256), dim3(256), 0, h->stream, g, st, inv2, r3h,
-                       h->spl_grad);
+    hipLaunchKernelGGL(k_spline_z<St>, dim3((lz + kSzLines - 1) / kSzLines), dim3(kSzLines), 0, h->stream, g, st,
+                       inv2, r3h, h->spl_grad);
     hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, h->stream, g, st, h->spl_grad, p,
                        h->red_part);
 }
