@@ -35,6 +35,8 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_VEC_TFLOPS = 78.6         # MI355X vector FP64
 FP32_VEC_TFLOPS = 157.3        # MI355X vector FP32
 ALG_BYTES_PER_CELL = 80.0      # 5 state words read + 5 written per cell-update, 8 B each
+VALU_SIMDS = 1024              # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9               # MI355X peak engine clock
 # measured once with rocprofv3 on the 256^3 workload (see the files for the commands)
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_flux_fp32_256_pmc.md"
 FLOPS_PROFILE = ROOT / "profiles" / "k_flux_256_valu_flops.md"
@@ -262,6 +264,8 @@ def main():
     traffic = profile_number(TRAFFIC_PROFILE, "traffic_bytes_per_launch") if n == 256 else None
     flops32 = profile_number(FLOPS_PROFILE, "flops_per_launch_fp32") if n == 256 else None
     flops64 = profile_number(FLOPS_PROFILE, "flops_per_launch_fp64") if n == 256 else None
+    vins32 = profile_number(FLOPS_PROFILE, "valu_instructions_per_launch_fp32") if n == 256 else None
+    vins64 = profile_number(FLOPS_PROFILE, "valu_instructions_per_launch_fp64") if n == 256 else None
     valu = None
     if flops32:
         tf = flops32 / (flux_ms * 1e-3) / 1e12
@@ -269,10 +273,15 @@ def main():
                 "frac": tf / FP32_VEC_TFLOPS, "flops_per_launch": flops32,
                 "note": "FP32 vector peak includes packed FP32; this build compiles with packed FP32 off, "
                         "so the reachable non-packed rate is half of it"}
+        if vins32:
+            # wave64 VALU instructions issued per second against 1024 SIMDs x 2.4 GHz / 4 cycles
+            valu["issue_frac"] = vins32 * 4 / (VALU_SIMDS * CLOCK_HZ * flux_ms * 1e-3)
         if res_fp64 and flops64:
             tf64 = flops64 / (res_fp64["flux_kernel_ms"] * 1e-3) / 1e12
             valu["fp64"] = {"achieved": tf64, "peak": FP64_VEC_TFLOPS, "frac": tf64 / FP64_VEC_TFLOPS,
                             "flops_per_launch": flops64}
+            if vins64:
+                valu["fp64"]["issue_frac"] = vins64 * 4 / (VALU_SIMDS * CLOCK_HZ * res_fp64["flux_kernel_ms"] * 1e-3)
 
     out = {
         "metric": "cell-updates/s, mixed vs FP64, 256^3 on 1 MI355X; % HBM roofline; E_k err",

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "gkm_erfcx.h"
+
 namespace gkm {
 
 template <class T> struct M;
@@ -28,7 +30,6 @@ template <class T> struct M;
 // refined by two Newton steps (within an ulp of 1/x for the normal-range
 // values that occur here; the IEEE division sequence is about twice as long).
 template <> struct M<double> {
-    __device__ static __forceinline__ double erfc_(double x) { return ::erfc(x); }
     __device__ static __forceinline__ double exp_(double x) { return ::exp(x); }
     __device__ static __forceinline__ double sqrt_(double x) { return ::sqrt(x); }
     __device__ static __forceinline__ double pow_(double x, double y) { return ::pow(x, y); }
@@ -39,15 +40,24 @@ template <> struct M<double> {
         e = fma(-x, r, 1.0);
         return fma(r, e, r);
     }
+    // 1/sqrt(x), x > 0 normal: v_rsq_f64 refined by two Newton steps
+    // r <- r (3 - x r^2)/2, each squaring the relative error
+    __device__ static __forceinline__ double rsqrt(double x) {
+        double r = __builtin_amdgcn_rsq(x);
+        const double hx = 0.5 * x;
+        r = r * fma(-hx * r, r, 1.5);
+        return r * fma(-hx * r, r, 1.5);
+    }
 };
 // FP32 (mixed form): hardware reciprocal, exponential and square root
-// (v_rcp_f32, v_exp_f32, v_sqrt_f32, about 1 ulp); erfc from the device library.
+// (v_rcp_f32, v_exp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp); erfc from the
+// erfcx polynomial below.
 template <> struct M<float> {
-    __device__ static __forceinline__ float erfc_(float x) { return ::erfcf(x); }
     __device__ static __forceinline__ float exp_(float x) { return __expf(x); }
     __device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ static __forceinline__ float pow_(float x, float y) { return ::powf(x, y); }
     __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 
 // ---------------------------------------------------------------------------
@@ -157,6 +167,8 @@ template <class T> struct Gas {
 // ---------------------------------------------------------------------------
 template <class T> struct Maxw {
     T den, U, V, W, lam, h;
+    T ri;  // 1/den
+    T P;   // pressure; h = 1/(2 lam) = P/den
 };
 
 template <class T> struct NTab {
@@ -168,12 +180,14 @@ template <class T>
 __device__ __forceinline__ void maxw_from_W(const Gas<T>& g, const T W[5], Maxw<T>& s) {
     s.den = W[0];
     const T ri = M<T>::rcp(W[0]);
+    s.ri = ri;
     s.U = W[1] * ri;
     s.V = W[2] * ri;
     s.W = W[3] * ri;
     const T P = (s.den * (g.gam - 1)) * (W[4] * ri - T(0.5) * (s.U * s.U + s.V * s.V + s.W * s.W));
+    s.P = P;
     s.lam = T(0.5) * s.den * M<T>::rcp(P);
-    s.h = M<T>::rcp(2 * s.lam);
+    s.h = P * ri;
 }
 
 template <class T> __device__ __forceinline__ void ntab_sums(const Gas<T>& g, T h, NTab<T>& t) {
@@ -195,13 +209,45 @@ template <class T> __device__ __forceinline__ void full_ntab(const Gas<T>& g, T 
     ntab_sums(g, h, t);
 }
 
+// erfcx(a) = exp(a^2) erfc(a), a >= 0, FP32: the polynomial of gkm_erfcx.h on
+// [0, kErfcxB]; beyond it (erfc < 2.2e-17) the asymptotic series
+// (1 - 1/(2a^2) + 3/(4a^4) - 15/(8a^6)) / (a sqrt(pi)).  Branch-free.
+template <class T> __device__ __forceinline__ T erfcx_tail(T a) {
+    const T ia = M<T>::rcp(a), i2 = ia * ia;
+    const T s = fma(fma(fma(T(-1.875), i2, T(0.75)), i2, T(-0.5)), i2, T(1));
+    return s * ia * T(0.56418958354775628694807945156077259);  // 1/sqrt(pi)
+}
+__device__ __forceinline__ float erfcx_(float a) {
+    const float u = fminf(a, float(kErfcxB)) * float(2.0 / kErfcxB) - 1.0f;
+    float p = kErfcxF[kErfcxDegF];
+#pragma unroll
+    for (int i = kErfcxDegF - 1; i >= 0; i--) p = fmaf(p, u, kErfcxF[i]);
+    return a > float(kErfcxB) ? erfcx_tail(a) : p;
+}
+
+// erfc(y) given e = exp(-y^2).  FP32: erfcx polynomial times e, branch-free.
+// FP64: the device library's erfc (a 35-term FP64 polynomial needs its
+// coefficients in registers, which the FP64 flux kernel, already at 256 VGPRs,
+// cannot spare without spilling).
+__device__ __forceinline__ float erfc_gauss(float y, float e) {
+    const float r = erfcx_(fabsf(y)) * e;
+    return y >= 0.0f ? r : 2.0f - r;
+}
+__device__ __forceinline__ double erfc_gauss(double y, double) { return ::erfc(y); }
+
 // half-space normal-axis table; sgn = +1 for u > 0, -1 for u < 0 (GKMflux.cpp:54-65)
 template <class T>
 __device__ __forceinline__ void half_ntab(const Gas<T>& g, T U, T lam, T h, int sgn, NTab<T>& t) {
-    const T sl = M<T>::sqrt_(lam);
-    const T pi = T(3.141592653589793238462643383279502884);
-    t.N[0] = T(0.5) * M<T>::erfc_(-T(sgn) * sl * U);
-    t.N[1] = U * t.N[0] + T(sgn) * T(0.5) * M<T>::exp_(-lam * U * U) * M<T>::rcp(M<T>::sqrt_(lam * pi));
+    // sqrt(lam) = lam / sqrt(lam) and 1/sqrt(lam pi) from one reciprocal square root
+    const T rs = M<T>::rsqrt(lam);
+    const T sl = lam * rs;
+    const T half_rsqrt_pi = T(0.28209479177387814347403972578038629);  // 1/(2 sqrt(pi))
+    // erfc(y), y = -sgn sqrt(lam) U, shares the Gaussian factor exp(-lam U^2)
+    // with N1: erfc(|y|) = erfcx(|y|) exp(-y^2) and erfc(-|y|) = 2 - erfc(|y|)
+    const T y = -T(sgn) * sl * U;
+    const T e = M<T>::exp_(-lam * U * U);
+    t.N[0] = T(0.5) * erfc_gauss(y, e);
+    t.N[1] = U * t.N[0] + T(sgn) * half_rsqrt_pi * e * rs;
 #pragma unroll
     for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + t.N[i - 2] * T(i - 1) * h;
     ntab_sums(g, h, t);
@@ -276,7 +322,7 @@ __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, co
                                            const T DWy[5], const T DWz[5], T c45, T c5, T p3, T p4, T p5, T by[5],
                                            T bz[5], T B[5], T F[5]) {
     const T h = s.h;
-    const T ri = M<T>::rcp(s.den);
+    const T ri = s.ri;
     T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = DWx[q] * ri;
@@ -354,9 +400,9 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     Maxw<T> l, r;
     maxw_from_W(g, Wl, l);
     maxw_from_W(g, Wr, r);
-    // collision time with the pressure-jump blending (GKMflux.cpp:149-150)
-    const T rl = l.den * M<T>::rcp(l.lam), rr = r.den * M<T>::rcp(r.lam);
-    const T tau = tau0 + dt * fabs(rl - rr) * M<T>::rcp(fabs(rl + rr));
+    // collision time with the pressure-jump blending (GKMflux.cpp:149-150);
+    // rho/lam = 2p, and the factor 2 cancels
+    const T tau = tau0 + dt * fabs(l.P - r.P) * M<T>::rcp(fabs(l.P + r.P));
     const T ex = M<T>::exp_(-dt * M<T>::rcp(tau));
     const T om = T(1) - ex;
     const T gm0 = dt - tau * om;
@@ -416,7 +462,7 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     // equilibrium state, in its own co-moving frame
     const T he = e.h;
     T axl[5], axr[5], ay[5], az[5], A[5], b[5], bp[5], sv[5];
-    const T s2 = T(2) * M<T>::rcp(e.den * dx);
+    const T s2 = T(2) * e.ri * M<T>::rcp(dx);
 #pragma unroll
     for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * s2;
     to_frame(b, e.V, e.W, bp);
@@ -432,10 +478,15 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
 #pragma unroll
     for (int q = 0; q < 5; q++) sv[q] = e.V * ay[q] + e.W * az[q];
     const T de = e.den;
-    NTab<T> ep, en, ef;
+    // The u < 0 half table is the full table minus the u > 0 one, and every
+    // moment is linear in the table, so <.. axl>_+ + <.. axr>_- is taken as
+    // <.. axr>_full + <.. (axl - axr)>_+ (one erfc/exp table fewer).
+    NTab<T> ep, ef;
     half_ntab(g, e.U, e.lam, he, +1, ep);
-    half_ntab(g, e.U, e.lam, he, -1, en);
     full_ntab(g, e.U, he, ef);
+    T dax[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) dax[q] = axl[q] - axr[q];
     // time-slope source of the equilibrium part, added in the frame
     to_frame(B, e.V, e.W, bp);
     {
@@ -445,12 +496,12 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
         for (int q = 0; q < 5; q++) c[q] = gm2 * sv[q];
         c[0] += gm1;
         acc_u<0>(bp, de, ef, he, c);
-        acc_u<1>(bp, dg2, ep, he, axl);
-        acc_u<1>(bp, dg2, en, he, axr);
+        acc_u<1>(bp, dg2, ef, he, axr);
+        acc_u<1>(bp, dg2, ep, he, dax);
         acc_v<0>(bp, dg2, ef, he, ay);
         acc_w<0>(bp, dg2, ef, he, az);
     }
-    const T ide = M<T>::rcp(de);
+    const T ide = e.ri;
 #pragma unroll
     for (int q = 0; q < 5; q++) bp[q] *= ide;
     solve_frame(g, bp, e.U, e.lam, he, A);
@@ -461,8 +512,8 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
         c[0] += p0;
         const T dp2 = de * p2;
         acc_u<1>(m, de, ef, he, c);
-        acc_u<2>(m, dp2, ep, he, axl);
-        acc_u<2>(m, dp2, en, he, axr);
+        acc_u<2>(m, dp2, ef, he, axr);
+        acc_u<2>(m, dp2, ep, he, dax);
         acc_v<1>(m, dp2, ef, he, ay);
         acc_w<1>(m, dp2, ef, he, az);
         add_to_lab(Fa, m, e.V, e.W);
