@@ -67,6 +67,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-realization ensemble")
     ap.add_argument("--no-slab", action="store_true", help="skip the 1024x1024x128 slab")
+    ap.add_argument("--no-gks", action="store_true", help="skip the smooth-flow GKS scheme")
     ap.add_argument("--slab-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
@@ -125,6 +126,24 @@ def cpu_baseline(args, gas_kw):
     return {"value": one, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle, {n}^3 box of the same recipe, {args.cpu_steps} steps, 1 thread",
             "omp_value": many, "omp_threads": threads, "cpu_model": cpu_model}
+
+
+def bench_gks(args, ic, dt):
+    """The GKM tree's smooth-flow scheme (interface average + central slopes,
+    one Maxwellian, no reconstruction) on the same box, mixed and FP64."""
+    import gkm_amd
+
+    n = ic.u.shape[0]
+    res = {}
+    for prec in ("mixed", "fp64"):
+        s = gkm_amd.Solver.from_ic(ic, precision=prec, visc_law="power", flux_scheme="gks", Pr=1.0)
+        sec, tim = time_solver(s, dt, args.steps, args.warmup)
+        s.close()
+        res[prec] = {"value": float(n) ** 3 * args.steps / sec, "ms_per_step": 1e3 * sec / args.steps,
+                     "flux_kernel_ms": tim["flux_ms"] / (3 * tim["steps"])}
+        print(f"[bench] gks {prec}: {1e3 * sec / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+    res["unit"] = "cell-updates/s"
+    return res
 
 
 def bench_ensemble(args):
@@ -313,6 +332,8 @@ def main():
         "mt_rel_err": mt_err,
         "cpu_baseline": None,
     }
+    if not args.no_gks:
+        out["gks"] = bench_gks(args, ic, dt)
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args)
     if not args.no_slab:

@@ -50,6 +50,8 @@ int check_params(const gkm_grid* g, const gkm_params* p) {
     if (p->recon != GKM_RECON_SYMOO && p->recon != GKM_RECON_JS) return fail(GKM_E_ARG, "bad recon");
     if (p->energy_recon != GKM_ENERGY_T && p->energy_recon != GKM_ENERGY_E) return fail(GKM_E_ARG, "bad energy_recon");
     if (p->precision != GKM_PREC_FP64 && p->precision != GKM_PREC_MIXED) return fail(GKM_E_ARG, "bad precision");
+    if (p->flux_scheme != GKM_SCHEME_WENO_BGK && p->flux_scheme != GKM_SCHEME_GKS)
+        return fail(GKM_E_ARG, "bad flux_scheme");
     return GKM_OK;
 }
 
@@ -117,8 +119,10 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
         int rc = check_params(grid, &params[b]);
         if (rc) return rc;
         if (params[b].recon != params[0].recon || params[b].energy_recon != params[0].energy_recon ||
-            params[b].zbug_compat != params[0].zbug_compat || params[b].precision != params[0].precision)
-            return fail(GKM_E_ARG, "box %d: recon, energy_recon, zbug_compat and precision must match box 0", b);
+            params[b].zbug_compat != params[0].zbug_compat || params[b].precision != params[0].precision ||
+            params[b].flux_scheme != params[0].flux_scheme)
+            return fail(GKM_E_ARG, "box %d: recon, energy_recon, zbug_compat, precision and flux_scheme must match box 0",
+                        b);
     }
     // 32-bit byte offsets address every plane of the batch
     const double padded = (double)(grid->nx + 6) * (grid->ny + 6) * (grid->nz + 6);
@@ -604,6 +608,23 @@ int gkm_weno_points(const gkm_params* p, int n, const double* in, double* out) {
         if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = fail(GKM_E_HIP, "weno points: %s", hipGetErrorString(e));
+    }
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int gkm_gks_points(const gkm_params* p, int n, const double* in, double* F) {
+    double *din = nullptr, *dout = nullptr;
+    hipStream_t s = nullptr;
+    int rc = points_common(p, n, in, 22, F, 5, &din, &dout, &s);
+    if (rc == GKM_OK && n > 0) {
+        gkm::launch_gks_points(s, p->precision, n, din, p->gamma, 1.0 / p->Pr, p->K, dout);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(F, dout, sizeof(double) * 5 * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(GKM_E_HIP, "gks points: %s", hipGetErrorString(e));
     }
     if (din) (void)hipFree(din);
     if (dout) (void)hipFree(dout);

@@ -522,4 +522,59 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     for (int q = 0; q < 5; q++) F[q] = Fa[q];
 }
 
+// Smooth-flow gas-kinetic flux of the GKM tree (GKM/CUDA GKM/src/GKMflux.cpp:12-280):
+// one Maxwellian at the interface average W0 with slopes ax, ay, az, the time
+// slope A from <psi (A.psi)> = -(<u psi ax> + <v psi ay> + <w psi az>), and
+//   F = rho0 ((p1 + p2) <u psi A> + p1 (<u^2 psi ax> + <uv psi ay> + <uw psi az>) + p0 <u psi>)
+// with p0 = dt, p1 = -tau dt, p2 = dt^2/2 (:269-278).  Taken in the Maxwellian's
+// tangentially co-moving frame like bgk_flux: v = v' + V and w = w' + W turn
+// <v psi ay> into <v' psi' ay> + V <psi' ay> (the sv terms).
+template <class T>
+__device__ __forceinline__ void gks_flux(const Gas<T>& g, const T W0[5], const T DWx[5], const T DWy[5],
+                                         const T DWz[5], T tau, T dt, T F[5]) {
+    Maxw<T> s;
+    maxw_from_W(g, W0, s);
+    const T h = s.h;
+    NTab<T> nf;
+    full_ntab(g, s.U, h, nf);
+    T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) b[q] = DWx[q] * s.ri;
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, ax);
+#pragma unroll
+    for (int q = 0; q < 5; q++) b[q] = DWy[q] * s.ri;
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, ay);
+#pragma unroll
+    for (int q = 0; q < 5; q++) b[q] = DWz[q] * s.ri;
+    to_frame(b, s.V, s.W, bp);
+    solve_frame(g, bp, s.U, s.lam, h, az);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        sv[q] = s.V * ay[q] + s.W * az[q];
+        b[q] = T(0);
+    }
+    acc_u<1>(b, T(-1), nf, h, ax);
+    acc_v<0>(b, T(-1), nf, h, ay);
+    acc_w<0>(b, T(-1), nf, h, az);
+    acc_u<0>(b, T(-1), nf, h, sv);
+    solve_frame(g, b, s.U, s.lam, h, A);
+    const T p0 = dt, p1 = -tau * dt, p2 = T(0.5) * dt * dt;
+    T c[5], m[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        c[q] = p1 * sv[q] + (p1 + p2) * A[q];
+        m[q] = T(0);
+        F[q] = T(0);
+    }
+    c[0] += p0;
+    const T dp1 = s.den * p1;
+    acc_u<1>(m, s.den, nf, h, c);
+    acc_u<2>(m, dp1, nf, h, ax);
+    acc_v<1>(m, dp1, nf, h, ay);
+    acc_w<1>(m, dp1, nf, h, az);
+    add_to_lab(F, m, s.V, s.W);
+}
+
 }  // namespace gkm

@@ -13,7 +13,7 @@
 // usage: gkm_dns <case-dir> [--proc-dim a b c] [--tmax T | --steps N]
 //        [--cfl C] [--precision fp64|mixed] [--recon symoo|js]
 //        [--visc sutherland|power] [--visc-exp e] [--energy T|E] [--pr Pr]
-//        [--R R] [--z-fixed] [--reinit DIR] [--timing]
+//        [--R R] [--z-fixed] [--scheme weno|gks] [--reinit DIR] [--timing]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -44,7 +44,7 @@ void usage() {
                  "usage: gkm_dns <case-dir> [--proc-dim a b c] [--tmax T | --steps N] [--cfl C]\n"
                  "               [--precision fp64|mixed] [--recon symoo|js] [--visc sutherland|power]\n"
                  "               [--visc-exp e] [--energy T|E] [--pr Pr] [--R R] [--z-fixed]\n"
-                 "               [--reinit DIR] [--timing]\n");
+                 "               [--scheme weno|gks] [--reinit DIR] [--timing]\n");
     std::exit(2);
 }
 
@@ -102,6 +102,9 @@ int main(int argc, char** argv) {
             par.Pr = std::atof(next());
         } else if (s == "--R") {
             par.R = std::atof(next());
+        } else if (s == "--scheme") {
+            const std::string v = next();
+            par.flux_scheme = v == "gks" ? GKM_SCHEME_GKS : GKM_SCHEME_WENO_BGK;
         } else if (s == "--z-fixed") {
             par.zbug_compat = 0;
         } else if (s == "--reinit") {

@@ -80,6 +80,8 @@ void launch_split(Ctx* h, int box, int q, const double* dsrc);
 void launch_merge(Ctx* h, int box, int q, double* ddst);
 void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,
                         double* out);
+void launch_gks_points(hipStream_t s, int prec, int n, const double* rec, double gam, double invPr, int K,
+                       double* out);
 void launch_weno_points(hipStream_t s, int prec, int recon, int n, const double* rec, double gam, double R,
                         int energy_recon, double* out);
 

@@ -287,6 +287,55 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
 }
 
 // ---------------------------------------------------------------------------
+// Sweep of the GKM tree's smooth-flow scheme (GKM/CUDA GKM/src/CPUupdate.cpp:
+// 29-192) on the core interface set: W0 = (W_i + W_i+1)/2, DWx = (W_i+1 -
+// W_i)/dx, DWy / DWz = half the difference of the neighbour rows' interface
+// averages along the tangential axes paired with the y / z slots, mu(T) from
+// the interface average (:62-65).  The two cells of the interface are read
+// in FP64 (T, p and tau are recovered in FP64 in both forms); the tangential
+// neighbours only enter slope differences and are read as cell values.
+// ---------------------------------------------------------------------------
+template <class Real, class St, int DIR>
+__global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st,
+                                             Planes5<Real> F, const StepPar* __restrict__ pars) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    Gas<Real> gas;
+    gas.gam = (Real)par.gam;
+    gas.invPr = (Real)par.invPr;
+    gas.K = (Real)par.K;
+    gas.kp3 = (Real)(par.K + 3);
+    gas.ikp3 = (Real)(1.0 / (par.K + 3));
+    const double idx = 1.0 / par.dx;
+    const Real hidx = Real(0.5 * idx);
+    double w0[5];  // interface average, field order
+    Real W0[5], Dx[5], Dy[5], Dz[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int q = slot_field<DIR>(s);
+        const double wl = st.get(q, c), wr = st.get(q, c + sn);
+        w0[q] = 0.5 * (wr + wl);
+        W0[s] = (Real)w0[q];
+        Dx[s] = (Real)((wr - wl) * idx);
+        const Real n1 = Real(0.5) * ((Real)st.cell(q, c + st1 + sn) + (Real)st.cell(q, c + st1));
+        const Real s1 = Real(0.5) * ((Real)st.cell(q, c - st1 + sn) + (Real)st.cell(q, c - st1));
+        const Real n2 = Real(0.5) * ((Real)st.cell(q, c + st2 + sn) + (Real)st.cell(q, c + st2));
+        const Real s2 = Real(0.5) * ((Real)st.cell(q, c - st2 + sn) + (Real)st.cell(q, c - st2));
+        Dy[s] = (n1 - s1) * hidx;
+        Dz[s] = (n2 - s2) * hidx;
+    }
+    const double r = w0[0], u = w0[1] / r, v = w0[2] / r, w = w0[3] / r;
+    const double P = (r * (par.gam - 1)) * (w0[4] / r - 0.5 * (u * u + v * v + w * w));
+    const double tau = viscosity(par, P / (par.R * r)) / P;
+    Real Fs[5];
+    gks_flux(gas, W0, Dx, Dy, Dz, (Real)tau, (Real)par.dt, Fs);
+#pragma unroll
+    for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
+}
+
+// ---------------------------------------------------------------------------
 // Conservative update on the interior (update.cpp:455-469): the flux
 // differences are summed in the reference's order.
 // ---------------------------------------------------------------------------
@@ -374,6 +423,32 @@ __global__ void k_flux_points(int n, const double* rec, double dx, double gam, d
     Real F[5];
     bgk_flux(gas, v[0], v[1], v[2], v[3], slopes, (Real)x[50], (Real)x[51], (Real)dx, F);
     for (int q = 0; q < 5; q++) out[(long)t * 5 + q] = (double)F[q];
+}
+
+// GKS flux of single records: W0[5] DWx[5] DWy[5] DWz[5] mu dt (slot order)
+template <class Real>
+__global__ void k_gks_points(int n, const double* rec, double gam, double invPr, int K, double* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double* x = rec + (size_t)t * 22;
+    Gas<Real> gas;
+    gas.gam = (Real)gam;
+    gas.invPr = (Real)invPr;
+    gas.K = (Real)K;
+    gas.kp3 = (Real)(K + 3);
+    gas.ikp3 = (Real)(1.0 / (K + 3));
+    Real W0[5], Dx[5], Dy[5], Dz[5], F[5];
+    for (int q = 0; q < 5; q++) {
+        W0[q] = (Real)x[q];
+        Dx[q] = (Real)x[5 + q];
+        Dy[q] = (Real)x[10 + q];
+        Dz[q] = (Real)x[15 + q];
+    }
+    // tau = mu/p of the interface average (GKMflux.cpp:47), in FP64
+    const double r = x[0], u = x[1] / r, v = x[2] / r, w = x[3] / r;
+    const double P = (r * (gam - 1)) * (x[4] / r - 0.5 * (u * u + v * v + w * w));
+    gks_flux(gas, W0, Dx, Dy, Dz, (Real)(x[20] / P), (Real)x[21], F);
+    for (int q = 0; q < 5; q++) out[(size_t)t * 5 + q] = (double)F[q];
 }
 
 template <class Real, int RECON>
@@ -489,11 +564,25 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     tangential_axes(dir, h->par.zbug_compat, t1, t2);
     const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
     Box3 bc = sweep_box(g, an, false);
-    if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_recon_any<Real, St>(h, dir, st, Tw, wi);
-    if (ev) (void)hipEventRecord(ev[1], h->stream);
     const int bs = h->flux_block;
     const dim3 grid(blocks_for(bc.count(), bs), h->nbox);
+    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction: one kernel per sweep
+        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        if (dir == 0)
+            hipLaunchKernelGGL((k_gks<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, F,
+                               h->d_pars);
+        else if (dir == 1)
+            hipLaunchKernelGGL((k_gks<Real, St, 1>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, F,
+                               h->d_pars);
+        else
+            hipLaunchKernelGGL((k_gks<Real, St, 2>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, F,
+                               h->d_pars);
+        if (ev) (void)hipEventRecord(ev[2], h->stream);
+        return;
+    }
+    launch_recon_any<Real, St>(h, dir, st, Tw, wi);
+    if (ev) (void)hipEventRecord(ev[1], h->stream);
     if (dir == 0)
         hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
@@ -587,10 +676,12 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
 
 void launch_step(Ctx* h, hipEvent_t* ev) {
     const Geo& g = h->geo;
+    // the GKS scheme reads no per-cell primitive planes
+    const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_ghost(h, true);
+    launch_ghost(h, prim);
     if (ev) (void)hipEventRecord(ev[12], h->stream);
-    if (h->halo_mode == GKM_HALO_PACK) launch_prim(h);
+    if (prim && h->halo_mode == GKM_HALO_PACK) launch_prim(h);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     Box3 bi = interior_box(g);
     const dim3 ugrid(blocks_for(bi.count(), 256), h->nbox);
@@ -649,6 +740,14 @@ void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, doubl
     else
         hipLaunchKernelGGL(k_flux_points<float>, dim3(blocks_for(n, 64)), dim3(64), 0, s, n, rec, dx, gam, invPr, K,
                            out);
+}
+
+void launch_gks_points(hipStream_t s, int prec, int n, const double* rec, double gam, double invPr, int K,
+                       double* out) {
+    if (prec == GKM_PREC_FP64)
+        hipLaunchKernelGGL(k_gks_points<double>, dim3(blocks_for(n, 64)), dim3(64), 0, s, n, rec, gam, invPr, K, out);
+    else
+        hipLaunchKernelGGL(k_gks_points<float>, dim3(blocks_for(n, 64)), dim3(64), 0, s, n, rec, gam, invPr, K, out);
 }
 
 void launch_weno_points(hipStream_t s, int prec, int recon, int n, const double* rec, double gam, double R,

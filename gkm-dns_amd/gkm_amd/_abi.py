@@ -23,6 +23,7 @@ VISC = {"sutherland": 0, "power": 1}
 RECON = {"symoo": 0, "js": 1}
 ENERGY = {"T": 0, "E": 1}
 PRECISION = {"fp64": 0, "mixed": 1}
+SCHEME = {"weno_bgk": 0, "gks": 1}
 
 DP = C.POINTER(C.c_double)
 
@@ -33,7 +34,7 @@ EXPORTS = (
     "gkm_weno_points", "gkm_last_error", "gkm_version",
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
-    "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats",
+    "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats", "gkm_gks_points",
     # include/gkm_io.h
     "gkm_read_weno_params", "gkm_write_weno_params", "gkm_read_weno_init", "gkm_write_weno_init",
     "gkm_read_init_dat", "gkm_write_init_dat", "gkm_init_dat_state",
@@ -51,6 +52,7 @@ class GkmParams(C.Structure):
         ("gamma", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu0", C.c_double),
         ("T0", C.c_double), ("visc_exp", C.c_double), ("K", C.c_int), ("visc_law", C.c_int),
         ("recon", C.c_int), ("energy_recon", C.c_int), ("zbug_compat", C.c_int), ("precision", C.c_int),
+        ("flux_scheme", C.c_int),
     ]
 
 
@@ -125,6 +127,7 @@ def load() -> C.CDLL:
     L.gkm_last_timing.argtypes = [vp, DP, C.POINTER(C.c_int)]
     L.gkm_flux_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
     L.gkm_weno_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
+    L.gkm_gks_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
     L.gkm_create_batch.argtypes = [C.POINTER(GkmGrid), C.POINTER(GkmParams), C.c_int, C.POINTER(vp)]
     L.gkm_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
     L.gkm_set_state_box.argtypes = [vp, C.c_int, C.POINTER(DP)]

@@ -42,6 +42,7 @@ class GasParams:
     zbug_compat: bool = True  # reference Z-sweep tangential pairing
     precision: str = "fp64"   # "fp64" or "mixed"
     K: int | None = None      # internal DOF; default int((5 - 3 gamma)/(gamma - 1))
+    flux_scheme: str = "weno_bgk"  # "weno_bgk" (weno tree) or "gks" (GKM tree smooth-flow scheme)
 
     def dof(self) -> int:
         if self.K is not None:
@@ -53,7 +54,7 @@ class GasParams:
             gamma=self.gamma, R=self.R, Pr=self.Pr, mu0=self.mu0, T0=self.T0, visc_exp=self.visc_exp,
             K=self.dof(), visc_law=_abi.VISC[self.visc_law], recon=_abi.RECON[self.recon],
             energy_recon=_abi.ENERGY[self.energy_recon], zbug_compat=int(bool(self.zbug_compat)),
-            precision=_abi.PRECISION[self.precision])
+            precision=_abi.PRECISION[self.precision], flux_scheme=_abi.SCHEME[self.flux_scheme])
 
 
 @dataclass
@@ -96,13 +97,13 @@ class Solver:
     @classmethod
     def from_ic(cls, ic: DhitIC, precision: str = "fp64", visc_law: str = "power", recon: str = "symoo",
                 energy_recon: str = "T", zbug_compat: bool = True, Pr: float = 0.7, R: float = 287.0,
-                visc_exp: float = 0.76) -> "Solver":
+                visc_exp: float = 0.76, flux_scheme: str = "weno_bgk") -> "Solver":
         """Solver on the IC's grid with mu0 from the IC and T0 = p0/(R rho0)
         (weno/src/FlowField.cpp:46)."""
         ny, nx, nz = ic.u.shape
         gas = GasParams(gamma=ic.gamma, R=R, Pr=Pr, mu0=ic.mu0, T0=ic.p0 / (R * ic.rho0), visc_law=visc_law,
                         visc_exp=visc_exp, recon=recon, energy_recon=energy_recon, zbug_compat=zbug_compat,
-                        precision=precision)
+                        precision=precision, flux_scheme=flux_scheme)
         s = cls(nx, ny, nz, gas=gas)
         s.initialize_dhit(ic)
         return s
@@ -381,6 +382,16 @@ def flux_points(gas: GasParams, rec: np.ndarray, dx: float) -> np.ndarray:
     p = gas.to_c()
     _abi.check(L.gkm_flux_points(C.byref(p), rec.shape[0], rec.ctypes.data_as(_abi.DP), float(dx),
                                  out.ctypes.data_as(_abi.DP)))
+    return out
+
+
+def gks_points(gas: GasParams, rec: np.ndarray) -> np.ndarray:
+    """Device GKS flux on (n, 22) records W0 DWx DWy DWz mu dt -> (n, 5)."""
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    out = np.zeros((rec.shape[0], 5))
+    p = gas.to_c()
+    _abi.check(_abi.load().gkm_gks_points(C.byref(p), rec.shape[0], rec.ctypes.data_as(_abi.DP),
+                                          out.ctypes.data_as(_abi.DP)))
     return out
 
 

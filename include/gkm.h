@@ -49,6 +49,12 @@ extern "C" {
 #define GKM_PREC_FP64  0       /* FP64 everywhere (drop-in form)                               */
 #define GKM_PREC_MIXED 1       /* FP32 reconstruction + flux, FP32 hi/lo compensated state     */
 
+#define GKM_SCHEME_WENO_BGK 0  /* WENO reconstruction + BGK-NS flux with half-space moments
+                                  (weno/src/update.cpp:194-469, GKMflux.cpp:11-215)              */
+#define GKM_SCHEME_GKS      1  /* smooth-flow gas-kinetic flux of the GKM tree: interface
+                                  average and central-difference slopes, one Maxwellian
+                                  (GKM/CUDA GKM/src/CPUupdate.cpp:29-210, GKMflux.cpp:12-280)    */
+
 /* Grid of one box.  The reference uses one spacing for all axes
  * (dx = 2*pi/nc, weno/src/FlowField.cpp:45). */
 typedef struct gkm_grid {
@@ -75,6 +81,8 @@ typedef struct gkm_params {
                             z-slot slope with d/dy, as update.cpp:380-423 does;
                             0: physically consistent pairing                  */
     int    precision;    /* GKM_PREC_*                                        */
+    int    flux_scheme;  /* GKM_SCHEME_* (0, the weno tree's scheme, when the
+                            struct is zero-initialised)                       */
 } gkm_params;
 
 /* Statistics of the interior cells, all accumulated in FP64.
@@ -138,7 +146,7 @@ int gkm_destroy(gkm_t* h);
 /* A batch of nbox independent boxes of one grid, advanced by the same
  * launches (one launch per stage covers every box).  params[b] gives box b's
  * gas constants (gamma, R, Pr, mu0, T0, visc_exp, K, visc_law may differ);
- * recon, energy_recon, zbug_compat and precision must agree.  No reference
+ * recon, energy_recon, zbug_compat, precision and flux_scheme must agree.  No reference
  * counterpart: the reference runs one segment per MPI rank.  The *_box calls
  * address one box; the single-box calls above act on box 0. */
 int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, gkm_t** out);
@@ -230,6 +238,10 @@ int gkm_set_halo_mode(gkm_t* h, int mode);
  *          out[n*10] = Wl[5] Wr[5]                                         */
 int gkm_flux_points(const gkm_params* p, int n, const double* in, double dx, double* F);
 int gkm_weno_points(const gkm_params* p, int n, const double* in, double* out);
+/* The GKM tree's smooth-flow flux on single records (replaces flux(),
+ * GKM/CUDA GKM/src/GKMflux.cpp:12-280, same argument list):
+ *   in[n*22] = W0[5] DWx[5] DWy[5] DWz[5] mu dt (rotated slot order);  F[n*5] */
+int gkm_gks_points(const gkm_params* p, int n, const double* in, double* F);
 
 const char* gkm_last_error(void);
 

@@ -389,6 +389,48 @@ inline double viscosity(const gkm_params& gp, double T) {
     return gp.mu0 * std::pow((T / gp.T0), 1.5) * (gp.T0 + 110.4) / (T + 110.4);
 }
 
+/* Smooth-flow gas-kinetic flux of the GKM tree (GKM/CUDA GKM/src/GKMflux.cpp:12-280):
+ * one Maxwellian at the interface average W0, slopes from the central
+ * differences, the time slope from the compatibility condition, and
+ *   F = rho0 ((p1 + p2) <u psi A> + p1 (<u^2 psi ax> + <uv psi ay> + <uw psi az>) + p0 <u psi>)
+ * with p0 = dt, p1 = -tau dt, p2 = dt^2/2, tau = mu/p0 (:47, :269-278).  The
+ * moments are the full-space ones of all three axes (:28-42). */
+void gks_flux(const gkm_params& gp, const double W0[5], const double DWx[5], const double DWy[5],
+              const double DWz[5], double mu, double dt, double F[5]) {
+    State s;
+    make_state(gp.gamma, gp.K, W0, s, false, false);
+    const double den0 = s.den, lam0 = s.lam;
+    const double* f0 = s.ax[0].fu;
+    const double* f1 = s.ax[1].fu;
+    const double* f2 = s.ax[2].fu;
+    const double tau = mu / s.P;
+    const double invPr = 1.0 / gp.Pr;
+    double bx[5], by[5], bz[5], ax[5], ay[5], az[5];
+    for (int i = 0; i < 5; i++) {
+        bx[i] = DWx[i] / den0;
+        by[i] = DWy[i] / den0;
+        bz[i] = DWz[i] / den0;
+    }
+    slopesolver(gp.K, invPr, bx, s.U, lam0, ax);
+    slopesolver(gp.K, invPr, by, s.U, lam0, ay);
+    slopesolver(gp.K, invPr, bz, s.U, lam0, az);
+    double Mu_ax[5], Mu2_ax[5], Muv_ay[5], Muw_az[5], Mv_ay[5], Mw_az[5], M[5], Mu_A[5], B[5], A[5];
+    mom_a(Mu_ax, f0, f1, f2, s.e2, s.e4, 1, 0, 0, ax);
+    mom_a(Mu2_ax, f0, f1, f2, s.e2, s.e4, 2, 0, 0, ax);
+    mom_a(Muv_ay, f0, f1, f2, s.e2, s.e4, 1, 1, 0, ay);
+    mom_a(Muw_az, f0, f1, f2, s.e2, s.e4, 1, 0, 1, az);
+    mom_a(Mv_ay, f0, f1, f2, s.e2, s.e4, 0, 1, 0, ay);
+    mom_a(Mw_az, f0, f1, f2, s.e2, s.e4, 0, 0, 1, az);
+    mom(M, f0, f1, f2, s.e2, 1, 0, 0);
+    for (int i = 0; i < 5; i++) B[i] = -Mu_ax[i] - Mv_ay[i] - Mw_az[i];
+    slopesolver(gp.K, invPr, B, s.U, lam0, A);
+    mom_a(Mu_A, f0, f1, f2, s.e2, s.e4, 1, 0, 0, A);
+    const double C0 = den0;
+    const double p0 = dt, p1 = -tau * dt, p2 = dt * dt / 2;
+    for (int i = 0; i < 5; i++)
+        F[i] = C0 * ((p1 + p2) * Mu_A[i] + p1 * Mu2_ax[i] + p1 * Muv_ay[i] + p1 * Muw_az[i] + p0 * M[i]);
+}
+
 /* ---- whole-box helpers ----------------------------------------------------- */
 
 struct Box {
@@ -510,6 +552,56 @@ void sweep(const Box& B, const gkm_params& gp, double* const W[5], double dt, in
             }
 }
 
+/* One sweep of the GKM tree's scheme (GKM/CUDA GKM/src/CPUupdate.cpp:29-210):
+ * W0 = (W_i + W_i+1)/2, DWx = (W_i+1 - W_i)/dx, DWy / DWz = half the difference
+ * of the interface averages of the two neighbour rows along the tangential
+ * axes paired with the y / z slots (the same pairing, Z-sweep quirk included,
+ * as the weno sweeps), mu(T) of the interface average (:62-65); core set:
+ * normal [2, P-4], tangential [3, P-4]. */
+void sweep_gks(const Box& B, const gkm_params& gp, double* const W[5], double dt, int dir, double* const F[5],
+               int nthreads) {
+    const Sweep S = make_sweep(dir, gp.zbug_compat);
+    const long sn = B.s[S.an], st1 = B.s[S.t1], st2 = B.s[S.t2];
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = 3;
+        hi[a] = B.P[a] - 4;
+    }
+    lo[S.an] = 2;
+    const double dx = B.dx;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(static)
+    for (int j = lo[0]; j <= hi[0]; j++)
+        for (int i = lo[1]; i <= hi[1]; i++)
+            for (int k = lo[2]; k <= hi[2]; k++) {
+                const long c = B.at(j, i, k);
+                double W0[5], DWx[5], DWy[5], DWz[5];
+                for (int q = 0; q < 5; q++) {
+                    const double* w = W[q];
+                    W0[q] = .5 * (w[c + sn] + w[c]);
+                    const double W0N = .5 * (w[c + st1 + sn] + w[c + st1]);
+                    const double W0S = .5 * (w[c - st1 + sn] + w[c - st1]);
+                    const double W0F = .5 * (w[c + st2 + sn] + w[c + st2]);
+                    const double W0B = .5 * (w[c - st2 + sn] + w[c - st2]);
+                    DWx[q] = (w[c + sn] - w[c]) / dx;
+                    DWy[q] = 0.5 * (W0N - W0S) / dx;
+                    DWz[q] = 0.5 * (W0F - W0B) / dx;
+                }
+                const double deni = W0[0];
+                const double ui = W0[1] / deni, vi = W0[2] / deni, wi = W0[3] / deni;
+                const double Vsqr = (ui * ui + vi * vi + wi * wi);
+                const double Pi = (deni * (gp.gamma - 1)) * (W0[4] / deni - .5 * Vsqr);
+                const double mu = viscosity(gp, Pi / (gp.R * deni));
+                double r0[5], rx[5], ry[5], rz[5];
+                for (int s = 0; s < 5; s++) {
+                    const int q = S.slot[s];
+                    r0[s] = W0[q]; rx[s] = DWx[q]; ry[s] = DWy[q]; rz[s] = DWz[q];
+                }
+                double Fs[5];
+                gks_flux(gp, r0, rx, ry, rz, mu, dt, Fs);
+                for (int s = 0; s < 5; s++) F[S.slot[s]][c] = Fs[s];
+            }
+}
+
 void fill_ghosts(const Box& B, double* const W[5]) {
     /* per axis, over the full padded extent of the others (FlowField.cpp:115-275) */
     for (int a = 0; a < 3; a++) {
@@ -554,6 +646,11 @@ void orc_flux(const gkm_params* p, const double WL[5], const double WR[5], const
     bgk_flux(*p, WL, WR, Wl, Wr, DWxl, DWxr, DWyl, DWyr, DWzl, DWzr, tau, dt, dx, F);
 }
 
+void orc_gks_flux(const gkm_params* p, const double W0[5], const double DWx[5], const double DWy[5],
+                  const double DWz[5], double mu, double dt, double F[5]) {
+    gks_flux(*p, W0, DWx, DWy, DWz, mu, dt, F);
+}
+
 void orc_weno(const gkm_params* p, const double cells[6][5], double Wl[5], double Wr[5]) {
     weno_vec(*p, cells, Wl, Wr);
 }
@@ -584,7 +681,10 @@ void orc_fill_ghosts(const gkm_grid* g, double* const W[5]) { fill_ghosts(Box(*g
 void orc_fluxes(const gkm_grid* g, const gkm_params* p, double* const W[5], double dt, double* const F[15],
                 int nthreads) {
     Box B(*g);
-    for (int dir = 0; dir < 3; dir++) sweep(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+    for (int dir = 0; dir < 3; dir++) {
+        if (p->flux_scheme == GKM_SCHEME_GKS) sweep_gks(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+        else sweep(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+    }
 }
 
 void orc_step(const gkm_grid* g, const gkm_params* p, double* const W[5], double dt, int nsteps, int nthreads) {
@@ -594,7 +694,10 @@ void orc_step(const gkm_grid* g, const gkm_params* p, double* const W[5], double
     for (int q = 0; q < 15; q++) F[q] = &store[(size_t)q * B.n];
     for (int s = 0; s < nsteps; s++) {
         fill_ghosts(B, W);
-        for (int dir = 0; dir < 3; dir++) sweep(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+        for (int dir = 0; dir < 3; dir++) {
+            if (p->flux_scheme == GKM_SCHEME_GKS) sweep_gks(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+            else sweep(B, *p, W, dt, dir, F + 5 * dir, nthreads);
+        }
         update(B, W, F);
     }
     fill_ghosts(B, W);

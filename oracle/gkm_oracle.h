@@ -9,7 +9,9 @@
  * (weno/src/GKMflux.cpp:11-308), ApplyWENO SYMOO / JS
  * (weno/src/update.cpp:502-625, 1421-1513), derivsX/Y/Z + update
  * (update.cpp:201-469), PeriodicBC (FlowField.cpp:115-275), timestep
- * (update.cpp:1057-1079) and interior statistics.
+ * (update.cpp:1057-1079) and interior statistics; the GKM tree's smooth-flow
+ * scheme (GKM/CUDA GKM/src/CPUupdate.cpp:29-210, GKMflux.cpp:12-280) when
+ * params->flux_scheme is GKM_SCHEME_GKS.
  */
 #ifndef GKM_ORACLE_H
 #define GKM_ORACLE_H
@@ -28,6 +30,12 @@ void orc_flux(const gkm_params* p, const double WL[5], const double WR[5],
               const double DWyl[5], const double DWyr[5],
               const double DWzl[5], const double DWzr[5],
               double tau, double dt, double dx, double F[5]);
+
+/* Smooth-flow gas-kinetic flux of the GKM tree with its argument list
+ * (GKM/CUDA GKM/src/GKMflux.cpp:12-280): interface average W0 and slopes in
+ * the rotated slot order, viscosity mu, time step dt. */
+void orc_gks_flux(const gkm_params* p, const double W0[5], const double DWx[5], const double DWy[5],
+                  const double DWz[5], double mu, double dt, double F[5]);
 
 /* Vector WENO at one interface: cells (LLL, LL, L, R, RR, RRR) x 5 fields
  * -> Wl[5], Wr[5].  p->recon selects SYMOO / JS; p->energy_recon selects

@@ -29,6 +29,7 @@ class Params(C.Structure):
         ("gamma", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu0", C.c_double),
         ("T0", C.c_double), ("visc_exp", C.c_double), ("K", C.c_int), ("visc_law", C.c_int),
         ("recon", C.c_int), ("energy_recon", C.c_int), ("zbug_compat", C.c_int), ("precision", C.c_int),
+        ("flux_scheme", C.c_int),
     ]
 
 
@@ -73,6 +74,7 @@ def lib():
             raise RuntimeError(f"oracle not built: {ORACLE_SO} (run `make -C oracle`)")
         L = C.CDLL(str(ORACLE_SO))
         L.orc_flux.argtypes = [C.POINTER(Params)] + [DP] * 10 + [C.c_double] * 3 + [DP]
+        L.orc_gks_flux.argtypes = [C.POINTER(Params), DP, DP, DP, DP, C.c_double, C.c_double, DP]
         L.orc_weno.argtypes = [C.POINTER(Params), DP, DP, DP]
         L.orc_weno_scalar.argtypes = [C.c_int, DP, DP, DP]
         L.orc_moment_tables.argtypes = [C.c_double, C.c_double, C.c_int, DP, DP, DP, DP, DP]
@@ -124,7 +126,7 @@ def make_grid(nx, ny, nz, dx):
 
 def make_params(**kw) -> Params:
     from_dict = dict(gamma=1.4, R=287.0, Pr=0.7, mu0=1e-3, T0=1.0, visc_exp=0.76, K=2, visc_law=0,
-                     recon=0, energy_recon=0, zbug_compat=1, precision=0)
+                     recon=0, energy_recon=0, zbug_compat=1, precision=0, flux_scheme=0)
     from_dict.update(kw)
     return Params(**from_dict)
 
@@ -140,6 +142,20 @@ def flux_points(params: Params, rec: np.ndarray, dx: float) -> np.ndarray:
         v = [np.ascontiguousarray(x[5 * a:5 * a + 5]) for a in range(10)]
         F = np.zeros(5)
         L.orc_flux(C.byref(params), *[_p(t) for t in v], float(x[50]), float(x[51]), float(dx), _p(F))
+        out[r] = F
+    return out
+
+
+def gks_flux_points(params: Params, rec: np.ndarray) -> np.ndarray:
+    """rec: (n, 22) records W0 DWx DWy DWz mu dt (rotated slot order)."""
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    out = np.zeros((rec.shape[0], 5))
+    L = lib()
+    for r in range(rec.shape[0]):
+        x = rec[r]
+        v = [np.ascontiguousarray(x[5 * a:5 * a + 5]) for a in range(4)]
+        F = np.zeros(5)
+        L.orc_gks_flux(C.byref(params), *[_p(t) for t in v], float(x[20]), float(x[21]), _p(F))
         out[r] = F
     return out
 
