@@ -64,11 +64,13 @@ int check_box(Ctx* h, int box) {
 }
 
 void drop_graph(Ctx* h) {
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
-    h->graph_exec = nullptr;
-    h->graph = nullptr;
-    h->graph_ready = false;
+    for (int f = 0; f < 2; f++) {
+        if (h->graph_exec[f]) (void)hipGraphExecDestroy(h->graph_exec[f]);
+        if (h->graph[f]) (void)hipGraphDestroy(h->graph[f]);
+        h->graph_exec[f] = nullptr;
+        h->graph[f] = nullptr;
+        h->graph_ready[f] = false;
+    }
 }
 
 // Upload per-box step parameters for the given dt.  They go through pinned
@@ -157,7 +159,8 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
     const bool fp64 = params[0].precision == GKM_PREC_FP64;
     const size_t plane = up(n * (fp64 ? sizeof(double) : sizeof(float)));
-    const int nplanes = fp64 ? (5 + 10 + 15 + 2) : (10 + 10 + 15 + 2);
+    // state (x2: current and next), interface values, fluxes, T and tau
+    const int nplanes = fp64 ? (10 + 10 + 15 + 2) : (20 + 10 + 15 + 2);
     h->red_blocks = 2048;
     const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
     const size_t pars = up(sizeof(gkm::StepPar) * nbox);
@@ -177,6 +180,7 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     };
     if (fp64) {
         for (int q = 0; q < 5; q++) h->Wd[q] = (double*)take(plane);
+        for (int q = 0; q < 5; q++) h->Wd_nx[q] = (double*)take(plane);
         for (int q = 0; q < 10; q++) h->WId[q] = (double*)take(plane);
         for (int q = 0; q < 15; q++) h->Fd[q] = (double*)take(plane);
         h->Td = (double*)take(plane);
@@ -184,6 +188,8 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     } else {
         for (int q = 0; q < 5; q++) h->Whi[q] = (float*)take(plane);
         for (int q = 0; q < 5; q++) h->Wlo[q] = (float*)take(plane);
+        for (int q = 0; q < 5; q++) h->Whi_nx[q] = (float*)take(plane);
+        for (int q = 0; q < 5; q++) h->Wlo_nx[q] = (float*)take(plane);
         for (int q = 0; q < 10; q++) h->WIf[q] = (float*)take(plane);
         for (int q = 0; q < 15; q++) h->Ff[q] = (float*)take(plane);
         h->Tf = (float*)take(plane);
@@ -307,16 +313,21 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
             gkm::launch_step(h, h->ev_step + s * gkm::kEvPerStep);
             h->timed_steps = s + 1;
         } else if (h->graph_enabled) {
-            if (!h->graph_ready) {
+            // a captured step holds the buffer pointers it was captured with:
+            // one graph per starting buffer
+            const int f = h->flip;
+            const bool capture = !h->graph_ready[f];
+            if (capture) {
                 HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                gkm::launch_step(h, nullptr);
+                gkm::launch_step(h, nullptr);  // swaps the pointers for the work it records
                 hipGraph_t gr = nullptr;
                 HIP_TRY(hipStreamEndCapture(h->stream, &gr));
-                h->graph = gr;
-                HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-                h->graph_ready = true;
+                h->graph[f] = gr;
+                HIP_TRY(hipGraphInstantiate(&h->graph_exec[f], h->graph[f], nullptr, nullptr, 0));
+                h->graph_ready[f] = true;
             }
-            HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+            HIP_TRY(hipGraphLaunch(h->graph_exec[f], h->stream));
+            if (!capture && h->par.flux_scheme == GKM_SCHEME_WENO_BGK) gkm::swap_state(h);
         } else {
             gkm::launch_step(h, nullptr);
         }
@@ -399,6 +410,25 @@ int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
     ms[2] = ghost;
     ms[3] = upd;
     ms[4] = prim;
+    return GKM_OK;
+}
+
+int gkm_last_sweep_timing(gkm_t* hh, double* ms) {
+    Ctx* h = ctx(hh);
+    if (!h || !ms) return fail(GKM_E_ARG, "null argument");
+    for (int d = 0; d < 3; d++) ms[d] = -1.0;
+    if (!h->timing || h->timed_steps == 0) return GKM_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int d = 0; d < 3; d++) {
+        double sum = 0;
+        for (int s = 0; s < h->timed_steps; s++) {
+            hipEvent_t* ev = h->ev_step + s * gkm::kEvPerStep;
+            float t;
+            HIP_TRY(hipEventElapsedTime(&t, ev[2 + 3 * d], ev[3 + 3 * d]));
+            sum += t;
+        }
+        ms[d] = sum;
+    }
     return GKM_OK;
 }
 

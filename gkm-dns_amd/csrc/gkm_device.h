@@ -126,6 +126,108 @@ __device__ __forceinline__ void weno_js(T LLL, T LL, T L, T R, T RR, T RRR, T ep
     wr = (A0 * P0n + A1 * P1n + A2 * P2n) * M<T>::rcp(A0 + A1 + A2);
 }
 
+// The same two schemes for a thread that walks consecutive interfaces along a
+// line.  The stencils of the right value at interface i+1/2 are those of the
+// left value at i+3/2 (the cells i-1..i+3 shifted by one), so their
+// smoothness weights are carried instead of recomputed:
+//   JS:    1/(eps+IS)^2 of IS2n, IS1n, IS0n -> IS0p, IS1p, IS2p of the next interface
+//   SYMOO: IS and 1/(eps+IS) of the same three (IS3p, the fourth left stencil,
+//          is new cells and is computed each time)
+// js_pset / symoo_pset give the left-side weights of the first interface.
+template <class T> struct Carry {
+    T r[3];   // JS: squared reciprocals; SYMOO: reciprocals
+    T is[3];  // SYMOO: smoothness indicators (for the max that replaces IS3)
+};
+
+template <class T> __device__ __forceinline__ T is_sq(T a, T b) {
+    return T(13) / T(12) * a * a + T(0.25) * b * b;
+}
+
+// (written as the previous interface's right-side indicators are, term for
+// term, so a carried weight and a freshly computed one are the same bits and
+// the results do not depend on where a thread's run of interfaces starts)
+template <class T> __device__ __forceinline__ void js_pset(T LLL, T LL, T L, T R, T RR, T eps, Carry<T>& c) {
+    T r;
+    r = M<T>::rcp(eps + is_sq(LLL - 2 * LL + L, LLL - 4 * LL + 3 * L)); c.r[0] = r * r;
+    r = M<T>::rcp(eps + is_sq(LL - 2 * L + R, LL - R));               c.r[1] = r * r;
+    r = M<T>::rcp(eps + is_sq(RR - 2 * R + L, RR - 4 * R + 3 * L));   c.r[2] = r * r;
+}
+
+template <class T>
+__device__ __forceinline__ void weno_js_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
+                                              T& wr) {
+    const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
+    const T P0p = LLL * c13 - LL * c76 + L * c116;
+    const T P0n = R * c116 - RR * c76 + RRR * c13;
+    const T P1p = -LL * c16 + L * c56 + R * c13;
+    const T P1n = L * c13 + R * c56 - RR * c16;
+    T A0 = T(0.1) * c.r[0], A1 = T(0.6) * c.r[1], A2 = T(0.3) * c.r[2];
+    wl = (A0 * P0p + A1 * P1p + A2 * P1n) * M<T>::rcp(A0 + A1 + A2);
+    T r0, r1, r2;
+    r0 = M<T>::rcp(eps + is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R)); r0 = r0 * r0;
+    r1 = M<T>::rcp(eps + is_sq(L - 2 * R + RR, L - RR));                 r1 = r1 * r1;
+    r2 = M<T>::rcp(eps + is_sq(LL - 2 * L + R, LL - 4 * L + 3 * R));     r2 = r2 * r2;
+    A0 = T(0.1) * r0; A1 = T(0.6) * r1; A2 = T(0.3) * r2;
+    wr = (A0 * P0n + A1 * P1n + A2 * P1p) * M<T>::rcp(A0 + A1 + A2);
+    c.r[0] = r2;  // IS2n -> IS0p of the next interface
+    c.r[1] = r1;
+    c.r[2] = r0;
+}
+
+template <class T> __device__ __forceinline__ void symoo_pset(T LLL, T LL, T L, T R, T RR, T eps, Carry<T>& c) {
+    c.is[0] = is_sq(L - 2 * LL + LLL, -3 * L + 4 * LL - LLL);
+    c.is[1] = is_sq(R - 2 * L + LL, LL - R);
+    c.is[2] = is_sq(RR - 2 * R + L, RR - 4 * R + 3 * L);
+#pragma unroll
+    for (int k = 0; k < 3; k++) c.r[k] = M<T>::rcp(eps + c.is[k]);
+}
+
+template <class T>
+__device__ __forceinline__ void weno_symoo_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
+                                                 T& wr) {
+    const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
+    const T P0p = LLL * c13 - LL * c76 + L * c116;
+    const T P0n = RRR * c13 - RR * c76 + R * c116;
+    const T P1p = -LL * c16 + L * c56 + R * c13;
+    const T P1n = -RR * c16 + R * c56 + L * c13;
+    const T P2p = L * c13 + R * c56 - RR * c16;
+    const T P2n = R * c13 + L * c56 - LL * c16;
+    const T P3p = R * c116 - RR * c76 + RRR * c13;
+    const T P3n = L * c116 - LL * c76 + LLL * c13;
+    const T g0 = T(1) / T(20), g1 = T(9) / T(20);
+    // left value: IS0p..IS2p carried, IS3p from the new stencil, replaced by the max
+    T IS3p = is_sq(R - 2 * RR + RRR, -5 * R + 8 * RR - 3 * RRR);
+    IS3p = fmax(fmax(fmax(c.is[0], c.is[1]), c.is[2]), IS3p);
+    T A0 = g0 * c.r[0], A1 = g1 * c.r[1], A2 = g1 * c.r[2], A3 = g0 * M<T>::rcp(eps + IS3p);
+    wl = (A0 * P0p + A1 * P1p + A2 * P2p + A3 * P3p) * M<T>::rcp(A0 + A1 + A2 + A3);
+    const T IS0n = is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R);
+    const T IS1n = is_sq(RR - 2 * R + L, L - RR);
+    const T IS2n = is_sq(R - 2 * L + LL, -3 * R + 4 * L - LL);
+    T IS3n = is_sq(L - 2 * LL + LLL, -5 * L + 8 * LL - 3 * LLL);
+    IS3n = fmax(fmax(fmax(IS0n, IS1n), IS2n), IS3n);
+    const T r0 = M<T>::rcp(eps + IS0n), r1 = M<T>::rcp(eps + IS1n), r2 = M<T>::rcp(eps + IS2n);
+    A0 = g0 * r0; A1 = g1 * r1; A2 = g1 * r2; A3 = g0 * M<T>::rcp(eps + IS3n);
+    wr = (A0 * P0n + A1 * P1n + A2 * P2n + A3 * P3n) * M<T>::rcp(A0 + A1 + A2 + A3);
+    c.is[0] = IS2n; c.r[0] = r2;  // stencil i-1..i+1 -> IS0p of the next interface
+    c.is[1] = IS1n; c.r[1] = r1;
+    c.is[2] = IS0n; c.r[2] = r0;
+}
+
+template <int RECON, class T>
+__device__ __forceinline__ void weno_line_carry(const T c[6], T eps, Carry<T>& k, T& wl, T& wr) {
+    if (RECON == 1)
+        weno_js_carry(c[0], c[1], c[2], c[3], c[4], c[5], eps, k, wl, wr);
+    else
+        weno_symoo_carry(c[0], c[1], c[2], c[3], c[4], c[5], eps, k, wl, wr);
+}
+
+template <int RECON, class T> __device__ __forceinline__ void weno_pset(const T c[5], T eps, Carry<T>& k) {
+    if (RECON == 1)
+        js_pset(c[0], c[1], c[2], c[3], c[4], eps, k);
+    else
+        symoo_pset(c[0], c[1], c[2], c[3], c[4], eps, k);
+}
+
 template <int RECON, class T>
 __device__ __forceinline__ void weno_line(const T c[6], T eps, T& wl, T& wr) {
     if (RECON == 1)

@@ -30,11 +30,19 @@ struct Ctx {
     StepPar* d_pars = nullptr;
     StepPar* h_pars = nullptr;
     bool pars_valid = false;
-    // captured step
+    // captured steps, one per state buffer the step starts from (flip)
     bool graph_enabled = false;
-    bool graph_ready = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ready[2] = {false, false};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    // The WENO-BGK step reads the state from one buffer and writes the updated
+    // interior to the other (the Z sweep's flux kernel applies the update);
+    // Wd / Whi / Wlo always name the current state, the *_nx arrays the other
+    // buffer, and flip counts the swaps mod 2.
+    int flip = 0;
+    double* Wd_nx[5] = {};
+    float* Whi_nx[5] = {};
+    float* Wlo_nx[5] = {};
     // FP64 form
     double* Wd[5] = {};
     double* WId[10] = {};
@@ -75,7 +83,10 @@ int report_error(int code, const char* msg);
 
 StepPar step_par(const gkm_grid& grid, const gkm_params& p, double dt);
 void launch_ghost(Ctx* h, bool prim);  // prim = false: ghosts only
-void launch_step(Ctx* h, hipEvent_t* ev);  // ev: 12 events or nullptr
+// one step; for the WENO-BGK scheme it leaves the new state in the other
+// buffer and swaps the current-state pointers (swap_state)
+void launch_step(Ctx* h, hipEvent_t* ev);  // ev: 13 events or nullptr
+void swap_state(Ctx* h);
 void launch_split(Ctx* h, int box, int q, const double* dsrc);
 void launch_merge(Ctx* h, int box, int q, double* ddst);
 void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,

@@ -4,6 +4,8 @@
 // split/merge for the mixed form.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "gkm_internal.h"
 
 namespace gkm {
@@ -147,6 +149,23 @@ __device__ __forceinline__ void recon_one(const Real (&v)[6][6], Real eps, const
     }
 }
 
+// recon_one for a thread walking consecutive interfaces: smoothness weights of
+// the left value carried from the previous interface's right value
+template <int RECON, int ENERGY, class Real>
+__device__ __forceinline__ void recon_one_carry(const Real (&v)[6][6], Real eps, const StepPar& par,
+                                                Carry<Real> (&cw)[6], Real wl[5], Real wr[5]) {
+    constexpr int nlin = ENERGY == 1 ? 5 : 4;
+#pragma unroll
+    for (int q = 0; q < nlin; q++) weno_line_carry<RECON>(v[q], eps, cw[q], wl[q], wr[q]);
+    if (ENERGY == 0) {
+        Real Tl, Tr;
+        weno_line_carry<RECON>(v[5], eps, cw[5], Tl, Tr);
+        const Real cR = Real(par.R / (par.gam - 1.0));
+        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
+        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
+    }
+}
+
 // Unit-stride sweep (Z): one interface per thread; neighbouring threads share
 // their six-cell stencils through the cache.
 template <class Real, class St, int RECON, int ENERGY>
@@ -176,8 +195,10 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st
 
 // Strided sweeps (X, Y): a thread walks CHUNK consecutive interfaces along the
 // sweep normal with a six-cell register window, so each cell is loaded once
-// per chunk instead of six times; threads run along z (unit stride) for
-// coalescing.  Box axes: n0 = chunks, n1 = the other tangential axis, n2 = z.
+// per chunk instead of six times, and carries the smoothness weights of each
+// interface's right value to the next interface's left value; threads run
+// along z (unit stride) for coalescing.  Box axes: n0 = chunks, n1 = the other
+// tangential axis, n2 = z.
 template <class Real, class St, int RECON, int ENERGY, int CHUNK>
 __global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, unsigned sa, int n_normal, St st,
                                                    const Real* __restrict__ Tw, Planes10<Real> wi,
@@ -206,18 +227,34 @@ __global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, u
     if (tmode)
 #pragma unroll
         for (int s = 0; s < 5; s++) v[5][s + 1] = ld(Tw, c + s * sn - 2 * sn);
+    // left-side weights of the first interface (its cells LLL..RR are v[.][1..5])
+    Carry<Real> cw[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+        if (q < nlin || (tmode && q == 5)) weno_pset<RECON>(&v[q][1], Real(1e-10), cw[q]);
+    // the cell entering the window next, loaded one iteration ahead so its
+    // latency hides behind the previous interface's arithmetic
+    Real nx[6];
+#pragma unroll
+    for (int q = 0; q < nlin; q++) nx[q] = (Real)st.cell(q, c + 3 * sn);
+    if (tmode) nx[5] = ld(Tw, c + 3 * sn);
 #pragma unroll 1
     for (int it = 0; it < count; it++) {
-        // slide: drop the oldest cell, load the one three cells ahead
+        // slide: drop the oldest cell, take the prefetched one, fetch the next
 #pragma unroll
         for (int q = 0; q < 6; q++)
 #pragma unroll
             for (int s = 0; s < 5; s++) v[q][s] = v[q][s + 1];
 #pragma unroll
-        for (int q = 0; q < nlin; q++) v[q][5] = (Real)st.cell(q, c + 3 * sn);
-        if (tmode) v[5][5] = ld(Tw, c + 3 * sn);
+        for (int q = 0; q < nlin; q++) v[q][5] = nx[q];
+        if (tmode) v[5][5] = nx[5];
+        if (it + 1 < count) {
+#pragma unroll
+            for (int q = 0; q < nlin; q++) nx[q] = (Real)st.cell(q, c + 4 * sn);
+            if (tmode) nx[5] = ld(Tw, c + 4 * sn);
+        }
         Real wl[5], wr[5];
-        recon_one<RECON, ENERGY>(v, Real(1e-10), par, wl, wr);
+        recon_one_carry<RECON, ENERGY>(v, Real(1e-10), par, cw, wl, wr);
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             put(wi.p[q], c, wl[q]);
@@ -243,14 +280,13 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 // needs 260 and runs at 1 wave, about 1.5x slower.  FP32 fits 3 waves without
 // a bound; forcing 4 spills.
 // ---------------------------------------------------------------------------
+// The flux of interface c in the rotated slot order (shared by k_flux and the
+// Z sweep's k_flux_update).
+
 template <class Real, class St, int DIR>
-__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
-    k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
-           Planes10<Real> wi, Planes5<Real> F, const StepPar* __restrict__ pars) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const StepPar& par = pars[blockIdx.y];
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+__device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, unsigned st2, const St& st,
+                                        const Real* __restrict__ tau, const Planes10<Real>& wi,
+                                        const StepPar& par, Real Fs[5]) {
     Gas<Real> gas;
     gas.gam = (Real)par.gam;
     gas.invPr = (Real)par.invPr;
@@ -280,8 +316,19 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
             Dz[s] = Real(0.5) * (ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2)) * idx;
         }
     };
-    Real Fs[5];
     bgk_flux(gas, WL, WR, Wl, Wr, slopes, tau0, (Real)par.dt, (Real)par.dx, Fs);
+}
+
+template <class Real, class St, int DIR>
+__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
+    k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
+           Planes10<Real> wi, Planes5<Real> F, const StepPar* __restrict__ pars) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    Real Fs[5];
+    flux_at<Real, St, DIR>(c, sn, st1, st2, st, tau, wi, par, Fs);
 #pragma unroll
     for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
@@ -343,14 +390,20 @@ __global__ void __launch_bounds__(256) k_update_d(Geo g, Box3 b, StateD S, Plane
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
     const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    // every load before the first store: the state planes may alias the flux
+    // planes as far as the compiler knows, so interleaving would serialise
+    // the five fields' memory round trips
+    double d[5], w[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double* Fx = F.p[q];
         const double* Fy = F.p[5 + q];
         const double* Fz = F.p[10 + q];
-        const double d = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
-        put(S.w[q], c, ld(S.w[q], c) - (1 / dx) * d);
+        d[q] = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
+        w[q] = ld(S.w[q], c);
     }
+#pragma unroll
+    for (int q = 0; q < 5; q++) put(S.w[q], c, w[q] - (1 / dx) * d[q]);
 }
 
 // Mixed form: the FP32 flux divergence is added to the hi/lo pair with an
@@ -365,21 +418,106 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
     hi = h2;
 }
 
+// new value of interior cell c into the other buffer: x and y flux planes of
+// the earlier sweeps, z fluxes of the cell's right (Fz) and left (Fm) faces
+__device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateD& S, const StateD& N,
+                                            const Planes15<double>& F, const double Fz[5], const double Fm[5],
+                                            const StepPar& par) {
+    double d[5], w[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        d[q] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0) + Fz[q] - Fm[q];
+        w[q] = ld(S.w[q], c);
+    }
+    const double dx = par.dx;
+#pragma unroll
+    for (int q = 0; q < 5; q++) put(N.w[q], c, w[q] - (1 / dx) * d[q]);
+}
+__device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateM& S, const StateM& N,
+                                            const Planes15<float>& F, const float Fz[5], const float Fm[5],
+                                            const StepPar& par) {
+    float R[5], hi[5], lo[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        R[q] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0) + Fz[q] - Fm[q];
+        hi[q] = ld(S.hi[q], c);
+        lo[q] = ld(S.lo[q], c);
+    }
+    const float idx = (float)(1.0 / par.dx);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        two_sum_add(hi[q], lo[q], -idx * R[q]);
+        put(N.hi[q], c, hi[q]);
+        put(N.lo[q], c, lo[q]);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM S, Planes15<float> F, float idx) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
     const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    float R[5], hi[5], lo[5];  // all loads first (see k_update_d)
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const float* Fx = F.p[q];
         const float* Fy = F.p[5 + q];
         const float* Fz = F.p[10 + q];
-        const float R = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
-        float hi = ld(S.hi[q], c), lo = ld(S.lo[q], c);
-        two_sum_add(hi, lo, -idx * R);
-        put(S.hi[q], c, hi);
-        put(S.lo[q], c, lo);
+        R[q] = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
+        hi[q] = ld(S.hi[q], c);
+        lo[q] = ld(S.lo[q], c);
     }
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        two_sum_add(hi[q], lo[q], -idx * R[q]);
+        put(S.hi[q], c, hi[q]);
+        put(S.lo[q], c, lo[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Z sweep fused with the conservative update.  Interfaces are numbered along
+// the z lines of the interior (j, i) columns, interface k+1/2 for k in
+// [2, P2-4] (nI = nz+1 per line), flattened with k fastest.  Wavefront w
+// takes the 64 consecutive interfaces starting at 63 w, so consecutive
+// wavefronts share one: lane l computes the flux at its interface, takes its
+// left neighbour's from lane l-1 (__shfl_up), and for l >= 1 and k >= 3
+// updates cell k from the x, y fluxes of the earlier sweeps and the two z
+// fluxes, in the summation order of k_update_d / k_update_m.  The state is
+// read from the current buffer and the new interior written to the other
+// one, so no lane's read of a neighbour cell can see an updated value.
+// ---------------------------------------------------------------------------
+struct ZLines {
+    unsigned count;  // interfaces of one box: lines * nI
+    FastDiv fnI, fnx;
+};
+
+template <class Real, class St>
+__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
+    k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
+                  Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars) {
+    const unsigned gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned e = (gt >> 6) * 63u + lane;
+    const bool valid = e < zl.count;
+    const StepPar& par = pars[blockIdx.y];
+    unsigned c = 0, kk = 0;
+    Real Fz[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
+    if (valid) {
+        const unsigned line = fdiv(e, zl.fnI);
+        kk = e - line * (zl.fnI.d);
+        const unsigned j = fdiv(line, zl.fnx);
+        const unsigned i = line - j * zl.fnx.d;
+        c = blockIdx.y * g.ncell + (j + 3) * g.s0 + (i + 3) * g.s1 + (kk + 2);
+        Real Fs[5];
+        flux_at<Real, St, 2>(c, 1u, st1, st2, st, tau, wi, par, Fs);
+#pragma unroll
+        for (int s = 0; s < 5; s++) Fz[slot_field<2>(s)] = Fs[s];
+    }
+    Real Fm[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) Fm[q] = __shfl_up(Fz[q], 1u, 64);
+    if (!valid || lane == 0 || kk == 0) return;
+    update_cell(g, c, st, nx, F, Fz, Fm, par);
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -595,6 +733,40 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
+// Z sweep of the WENO-BGK scheme: reconstruction, then the flux fused with
+// the update of every interior cell into the other state buffer
+template <class Real, class St>
+static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Real* tau, Planes10<Real> wi,
+                                  Planes15<Real> F, hipEvent_t* ev) {
+    const Geo& g = h->geo;
+    int t1, t2;
+    tangential_axes(2, h->par.zbug_compat, t1, t2);
+    const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
+    ZLines zl;
+    const unsigned nI = (unsigned)g.nz + 1;
+    zl.count = (unsigned)g.ny * (unsigned)g.nx * nI;
+    zl.fnI = make_fastdiv(nI);
+    zl.fnx = make_fastdiv((unsigned)g.nx);
+    const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
+    const int bs = h->flux_block;
+    const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
+    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    launch_recon_any<Real, St>(h, 2, st, Tw, wi);
+    if (ev) (void)hipEventRecord(ev[1], h->stream);
+    hipLaunchKernelGGL((k_flux_update<Real, St>), grid, dim3(bs), 0, h->stream, g, zl, st1, st2, st, nx, tau, wi, F,
+                       h->d_pars);
+    if (ev) (void)hipEventRecord(ev[2], h->stream);
+}
+
+void swap_state(Ctx* h) {
+    for (int q = 0; q < 5; q++) {
+        std::swap(h->Wd[q], h->Wd_nx[q]);
+        std::swap(h->Whi[q], h->Whi_nx[q]);
+        std::swap(h->Wlo[q], h->Wlo_nx[q]);
+    }
+    h->flip ^= 1;
+}
+
 void launch_ghost(Ctx* h, bool prim) {
     const Geo& g = h->geo;
     Box3 all = padded_box(g);
@@ -676,8 +848,10 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
 
 void launch_step(Ctx* h, hipEvent_t* ev) {
     const Geo& g = h->geo;
-    // the GKS scheme reads no per-cell primitive planes
+    // the GKS scheme reads no per-cell primitive planes; the WENO-BGK scheme
+    // fuses the update into its Z sweep and writes the other state buffer
     const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
+    const bool fused = h->par.flux_scheme == GKM_SCHEME_WENO_BGK;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     launch_ghost(h, prim);
     if (ev) (void)hipEventRecord(ev[12], h->stream);
@@ -690,14 +864,21 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
         Planes10<double> wi;
         for (int q = 0; q < 10; q++) wi.p[q] = h->WId[q];
-        for (int dir = 0; dir < 3; dir++) {
-            Planes5<double> F;
-            for (int q = 0; q < 5; q++) F.p[q] = h->Fd[5 * dir + q];
-            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, F, ev ? ev + 1 + 3 * dir : nullptr);
-        }
         Planes15<double> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Fd[q];
-        hipLaunchKernelGGL(k_update_d, ugrid, dim3(256), 0, h->stream, g, bi, st, F, h->grid.dx);
+        const int nsep = fused ? 2 : 3;  // sweeps with a separate flux kernel
+        for (int dir = 0; dir < nsep; dir++) {
+            Planes5<double> Fd;
+            for (int q = 0; q < 5; q++) Fd.p[q] = h->Fd[5 * dir + q];
+            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, Fd, ev ? ev + 1 + 3 * dir : nullptr);
+        }
+        if (fused) {
+            StateD nx;
+            for (int q = 0; q < 5; q++) nx.w[q] = h->Wd_nx[q];
+            launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr);
+        } else {
+            hipLaunchKernelGGL(k_update_d, ugrid, dim3(256), 0, h->stream, g, bi, st, F, h->grid.dx);
+        }
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
@@ -706,16 +887,27 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
         }
         Planes10<float> wi;
         for (int q = 0; q < 10; q++) wi.p[q] = h->WIf[q];
-        for (int dir = 0; dir < 3; dir++) {
-            Planes5<float> F;
-            for (int q = 0; q < 5; q++) F.p[q] = h->Ff[5 * dir + q];
-            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, F, ev ? ev + 1 + 3 * dir : nullptr);
-        }
         Planes15<float> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Ff[q];
-        hipLaunchKernelGGL(k_update_m, ugrid, dim3(256), 0, h->stream, g, bi, st, F, (float)(1.0 / h->grid.dx));
+        const int nsep = fused ? 2 : 3;
+        for (int dir = 0; dir < nsep; dir++) {
+            Planes5<float> Ff;
+            for (int q = 0; q < 5; q++) Ff.p[q] = h->Ff[5 * dir + q];
+            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, Ff, ev ? ev + 1 + 3 * dir : nullptr);
+        }
+        if (fused) {
+            StateM nx;
+            for (int q = 0; q < 5; q++) {
+                nx.hi[q] = h->Whi_nx[q];
+                nx.lo[q] = h->Wlo_nx[q];
+            }
+            launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr);
+        } else {
+            hipLaunchKernelGGL(k_update_m, ugrid, dim3(256), 0, h->stream, g, bi, st, F, (float)(1.0 / h->grid.dx));
+        }
     }
     if (ev) (void)hipEventRecord(ev[11], h->stream);
+    if (fused) swap_state(h);
 }
 
 void launch_split(Ctx* h, int box, int q, const double* dsrc) {

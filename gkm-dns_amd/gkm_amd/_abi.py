@@ -35,6 +35,7 @@ EXPORTS = (
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
     "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats", "gkm_gks_points",
+    "gkm_last_sweep_timing",
     # include/gkm_io.h
     "gkm_read_weno_params", "gkm_write_weno_params", "gkm_read_weno_init", "gkm_write_weno_init",
     "gkm_read_init_dat", "gkm_write_init_dat", "gkm_init_dat_state",
@@ -125,6 +126,7 @@ def load() -> C.CDLL:
     L.gkm_sync.argtypes = [vp]
     L.gkm_set_timing.argtypes = [vp, C.c_int]
     L.gkm_last_timing.argtypes = [vp, DP, C.POINTER(C.c_int)]
+    L.gkm_last_sweep_timing.argtypes = [vp, DP]
     L.gkm_flux_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
     L.gkm_weno_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
     L.gkm_gks_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]

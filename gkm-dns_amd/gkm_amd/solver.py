@@ -256,8 +256,10 @@ class Solver:
         ms = (C.c_double * 5)()
         n = C.c_int()
         _abi.check(self._L.gkm_last_timing(self._h, ms, C.byref(n)))
+        sw = (C.c_double * 3)()
+        _abi.check(self._L.gkm_last_sweep_timing(self._h, sw))
         return {"flux_ms": ms[0], "recon_ms": ms[1], "ghost_ms": ms[2], "update_ms": ms[3], "prim_ms": ms[4],
-                "steps": n.value}
+                "flux_x_ms": sw[0], "flux_y_ms": sw[1], "flux_z_ms": sw[2], "steps": n.value}
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
