@@ -273,16 +273,14 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 
 // ---------------------------------------------------------------------------
 // BGK flux of one sweep on the core interface set (normal [2, P-4],
-// tangential [3, P-4]); writes the five flux planes of that direction.
-// st1 / st2 are the strides of the tangential axes paired with the y / z
-// slots of the rotated frame.
+// tangential [3, P-4]).  flux_at gives the flux of interface c in the rotated
+// slot order; k_flux writes the five flux planes of its direction (X, Y), and
+// the Z sweep's k_flux_update below also applies the update.  st1 / st2 are
+// the strides of the tangential axes paired with the y / z slots.
 // FP64: at least 2 waves per SIMD (256 registers); the unconstrained build
 // needs 260 and runs at 1 wave, about 1.5x slower.  FP32 fits 3 waves without
 // a bound; forcing 4 spills.
 // ---------------------------------------------------------------------------
-// The flux of interface c in the rotated slot order (shared by k_flux and the
-// Z sweep's k_flux_update).
-
 template <class Real, class St, int DIR>
 __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, unsigned st2, const St& st,
                                         const Real* __restrict__ tau, const Planes10<Real>& wi,
