@@ -10,7 +10,9 @@ One "step" = ghost fill + X/Y/Z reconstruction and flux sweeps + update of the
 whole box.  The headline value is the mixed form (FP32 reconstruction/flux,
 FP32 hi/lo state); the FP64 drop-in form is timed in the same run on the same
 initial condition, and the relative E_k / Mt error of the mixed run against the
-FP64 run after the same steps is reported.
+FP64 run after the same steps is reported.  A decay run of both forms to
+t/t0 = 4 (field "long_run") reports the largest relative error of E_k, Mt, the
+dilatational dissipation and the velocity-derivative skewness over the run.
 
 The CPU baseline is the FP64 oracle (the restated reference CPU step) on a
 bounded sample: a 64^3 box of the same recipe for a few steps, single-threaded
@@ -68,6 +70,9 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-realization ensemble")
     ap.add_argument("--no-slab", action="store_true", help="skip the 1024x1024x128 slab")
     ap.add_argument("--no-gks", action="store_true", help="skip the smooth-flow GKS scheme")
+    ap.add_argument("--no-long", action="store_true", help="skip the mixed-vs-FP64 decay run to t/t0 = 4")
+    ap.add_argument("--long-t", type=float, default=4.0, help="end of the decay run in eddy-turnover times")
+    ap.add_argument("--long-every", type=int, default=25, help="steps between statistics samples")
     ap.add_argument("--slab-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
@@ -144,6 +149,31 @@ def bench_gks(args, ic, dt):
         print(f"[bench] gks {prec}: {1e3 * sec / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
     res["unit"] = "cell-updates/s"
     return res
+
+
+def bench_long_run(args, ic, kw):
+    """The mixed form against the FP64 GPU form over the decay to t/t0 = 4 on the
+    same box (E_k, Mt, dilatational dissipation, velocity-derivative skewness),
+    and the FP64 run's q^2/q^2(0) beside the GKM tree's published Mt 0.5 curve."""
+    import gkm_amd
+
+    r = gkm_amd.mixed_vs_fp64(ic, t_end=args.long_t, cfl=args.cfl, every=args.long_every, **kw)
+    print(f"[bench] decay run: {r['steps']} steps to t/t0 = {args.long_t} in {r['seconds']:.1f} s, "
+          f"max rel err {r['max_rel_err']}", file=sys.stderr, flush=True)
+    curve = []
+    for s in r["samples"]:
+        if not curve or s["t_over_t0"] >= curve[-1][0] + 0.25 or s is r["samples"][-1]:
+            f = s["fp64"]
+            curve.append([round(s["t_over_t0"], 4), f["ek"], f["mt"], f["diss_dil"], f["skew"]])
+    pub = gkm_amd.PUBLISHED_MT05_RE72
+    return {"t_end_over_t0": args.long_t, "steps": r["steps"], "dt": r["dt"], "sample_every": r["sample_every"],
+            "max_rel_err": r["max_rel_err"], "tolerance_ek_mt": 1e-3,
+            "curve_fp64": {"columns": ["t/t0", "ek", "mt", "diss_dil", "skew"], "rows": curve},
+            "q2_decay_fp64": {str(k): v for k, v in r["q2_decay"].items()},
+            "q2_decay_published": {"values": {str(k): v for k, v in pub.items()},
+                                   "source": "GKM/CUDA GKM/output_Mach-0.5-Re72.dat (GKM central scheme, "
+                                             "Mt 0.5, Re 72, its own initial condition)"},
+            "seconds": r["seconds"]}
 
 
 def bench_ensemble(args):
@@ -335,6 +365,8 @@ def main():
     }
     if not args.no_gks:
         out["gks"] = bench_gks(args, ic, dt)
+    if not args.no_fp64 and not args.no_long:
+        out["long_run"] = bench_long_run(args, ic, kw)
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args)
     if not args.no_slab:
