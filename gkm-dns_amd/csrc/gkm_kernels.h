@@ -104,9 +104,37 @@ template <class T> struct Planes15 {
     T* p[15];  // Fx[0..4], Fy[0..4], Fz[0..4]
 };
 
+// uniform values the kernels use in their arithmetic type, derived once per
+// box: the kernels then do no per-thread divisions or FP64 -> FP32 conversions
+// of them (each is the same bits as the expression computed in the kernel)
+enum : int { kUGam, kUInvPr, kUK, kUKp3, kUIkp3, kUIdx, kUDt, kUDx, kUCR, kNU };
+
 struct StepPar {
     double gam, R, invPr, mu0, T0, visc_exp, dt, dx;
     int K, visc_law, energy_recon;
+    double ud[kNU];  // gam, 1/Pr, K, K+3, 1/(K+3), 1/dx, dt, dx, R/(gam-1)
+    float uf[kNU];   // the same rounded to float
 };
+
+// fills ud / uf from the fields above
+__host__ __device__ inline void derive_uniforms(StepPar& p) {
+    p.ud[kUGam] = p.gam;
+    p.ud[kUInvPr] = p.invPr;
+    p.ud[kUK] = (double)p.K;
+    p.ud[kUKp3] = (double)(p.K + 3);
+    p.ud[kUIkp3] = 1.0 / (p.K + 3);
+    p.ud[kUIdx] = 1.0 / p.dx;
+    p.ud[kUDt] = p.dt;
+    p.ud[kUDx] = p.dx;
+    p.ud[kUCR] = p.R / (p.gam - 1.0);
+    for (int i = 0; i < kNU; i++) p.uf[i] = (float)p.ud[i];
+}
+
+template <class Real> __device__ __forceinline__ Real upar(const StepPar& p, int i) {
+    if constexpr (sizeof(Real) == 8)
+        return p.ud[i];
+    else
+        return p.uf[i];
+}
 
 }  // namespace gkm

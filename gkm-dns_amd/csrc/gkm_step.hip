@@ -143,7 +143,7 @@ __device__ __forceinline__ void recon_one(const Real (&v)[6][6], Real eps, const
     if (ENERGY == 0) {
         Real Tl, Tr;
         weno_line<RECON>(v[5], eps, Tl, Tr);
-        const Real cR = Real(par.R / (par.gam - 1.0));
+        const Real cR = upar<Real>(par, kUCR);
         wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
         wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
     }
@@ -160,7 +160,7 @@ __device__ __forceinline__ void recon_one_carry(const Real (&v)[6][6], Real eps,
     if (ENERGY == 0) {
         Real Tl, Tr;
         weno_line_carry<RECON>(v[5], eps, cw[5], Tl, Tr);
-        const Real cR = Real(par.R / (par.gam - 1.0));
+        const Real cR = upar<Real>(par, kUCR);
         wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
         wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
     }
@@ -286,12 +286,12 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
                                         const Real* __restrict__ tau, const Planes10<Real>& wi,
                                         const StepPar& par, Real Fs[5]) {
     Gas<Real> gas;
-    gas.gam = (Real)par.gam;
-    gas.invPr = (Real)par.invPr;
-    gas.K = (Real)par.K;
-    gas.kp3 = (Real)(par.K + 3);
-    gas.ikp3 = (Real)(1.0 / (par.K + 3));
-    const Real idx = Real(1.0 / par.dx);
+    gas.gam = upar<Real>(par, kUGam);
+    gas.invPr = upar<Real>(par, kUInvPr);
+    gas.K = upar<Real>(par, kUK);
+    gas.kp3 = upar<Real>(par, kUKp3);
+    gas.ikp3 = upar<Real>(par, kUIkp3);
+    const Real idx = upar<Real>(par, kUIdx);
 
     // collision time mu/p of the left cell (update.cpp:262-264), from the plane
     const Real tau0 = ld(tau, c);
@@ -314,7 +314,7 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
             Dz[s] = Real(0.5) * (ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2)) * idx;
         }
     };
-    bgk_flux(gas, WL, WR, Wl, Wr, slopes, tau0, (Real)par.dt, (Real)par.dx, Fs);
+    bgk_flux(gas, WL, WR, Wl, Wr, slopes, tau0, upar<Real>(par, kUDt), upar<Real>(par, kUDx), Fs);
 }
 
 template <class Real, class St, int DIR>
@@ -348,12 +348,12 @@ __global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigne
     const StepPar& par = pars[blockIdx.y];
     const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
     Gas<Real> gas;
-    gas.gam = (Real)par.gam;
-    gas.invPr = (Real)par.invPr;
-    gas.K = (Real)par.K;
-    gas.kp3 = (Real)(par.K + 3);
-    gas.ikp3 = (Real)(1.0 / (par.K + 3));
-    const double idx = 1.0 / par.dx;
+    gas.gam = upar<Real>(par, kUGam);
+    gas.invPr = upar<Real>(par, kUInvPr);
+    gas.K = upar<Real>(par, kUK);
+    gas.kp3 = upar<Real>(par, kUKp3);
+    gas.ikp3 = upar<Real>(par, kUIkp3);
+    const double idx = par.ud[kUIdx];
     const Real hidx = Real(0.5 * idx);
     double w0[5];  // interface average, field order
     Real W0[5], Dx[5], Dy[5], Dz[5];
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigne
     const double P = (r * (par.gam - 1)) * (w0[4] / r - 0.5 * (u * u + v * v + w * w));
     const double tau = viscosity(par, P / (par.R * r)) / P;
     Real Fs[5];
-    gks_flux(gas, W0, Dx, Dy, Dz, (Real)tau, (Real)par.dt, Fs);
+    gks_flux(gas, W0, Dx, Dy, Dz, (Real)tau, upar<Real>(par, kUDt), Fs);
 #pragma unroll
     for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
@@ -427,9 +427,9 @@ __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const Stat
         d[q] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0) + Fz[q] - Fm[q];
         w[q] = ld(S.w[q], c);
     }
-    const double dx = par.dx;
+    const double idx = par.ud[kUIdx];  // 1 / dx
 #pragma unroll
-    for (int q = 0; q < 5; q++) put(N.w[q], c, w[q] - (1 / dx) * d[q]);
+    for (int q = 0; q < 5; q++) put(N.w[q], c, w[q] - idx * d[q]);
 }
 __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateM& S, const StateM& N,
                                             const Planes15<float>& F, const float Fz[5], const float Fm[5],
@@ -441,7 +441,7 @@ __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const Stat
         hi[q] = ld(S.hi[q], c);
         lo[q] = ld(S.lo[q], c);
     }
-    const float idx = (float)(1.0 / par.dx);
+    const float idx = par.uf[kUIdx];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         two_sum_add(hi[q], lo[q], -idx * R[q]);
@@ -498,14 +498,22 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     const unsigned e = (gt >> 6) * 63u + lane;
     const bool valid = e < zl.count;
     const StepPar& par = pars[blockIdx.y];
-    unsigned c = 0, kk = 0;
-    Real Fz[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
+    unsigned c = 0;
+    bool upd = false;  // lane 0's cell belongs to the previous wavefront; k = 2 is a ghost
     if (valid) {
         const unsigned line = fdiv(e, zl.fnI);
-        kk = e - line * (zl.fnI.d);
+        const unsigned kk = e - line * (zl.fnI.d);
+        upd = lane != 0 && kk != 0;
         const unsigned j = fdiv(line, zl.fnx);
         const unsigned i = line - j * zl.fnx.d;
         c = blockIdx.y * g.ncell + (j + 3) * g.s0 + (i + 3) * g.s1 + (kk + 2);
+    }
+    // the flags as a wave-uniform mask (scalar registers), taken before the
+    // flux so that no per-lane index value stays live in a vector register
+    // across it
+    const unsigned long long upd_mask = __ballot(upd);
+    Real Fz[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
+    if (valid) {
         Real Fs[5];
         flux_at<Real, St, 2>(c, 1u, st1, st2, st, tau, wi, par, Fs);
 #pragma unroll
@@ -514,7 +522,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     Real Fm[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) Fm[q] = __shfl_up(Fz[q], 1u, 64);
-    if (!valid || lane == 0 || kk == 0) return;
+    if (!((upd_mask >> __lane_id()) & 1ull)) return;
     update_cell(g, c, st, nx, F, Fz, Fm, par);
 }
 
@@ -595,6 +603,9 @@ __global__ void k_weno_points(int n, const double* rec, double gam, double R, in
     StepPar par{};
     par.gam = gam;
     par.R = R;
+    par.K = 0;
+    par.dx = 1.0;
+    derive_uniforms(par);
     Real v[6][6];
     for (int q = 0; q < 5; q++)
         for (int s = 0; s < 6; s++) v[q][s] = (Real)x[s * 5 + q];
@@ -841,6 +852,7 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
     p.K = pp.K;
     p.visc_law = pp.visc_law;
     p.energy_recon = pp.energy_recon;
+    derive_uniforms(p);
     return p;
 }
 
