@@ -107,12 +107,12 @@ template <class T> struct Planes15 {
 // uniform values the kernels use in their arithmetic type, derived once per
 // box: the kernels then do no per-thread divisions or FP64 -> FP32 conversions
 // of them (each is the same bits as the expression computed in the kernel)
-enum : int { kUGam, kUInvPr, kUK, kUKp3, kUIkp3, kUIdx, kUDt, kUDx, kUCR, kNU };
+enum : int { kUGam, kUInvPr, kUK, kUKp3, kUIkp3, kUIdx, kUDt, kUDx, kUCR, kUIR, kUMu0, kUIT0, kUVexp, kNU };
 
 struct StepPar {
     double gam, R, invPr, mu0, T0, visc_exp, dt, dx;
     int K, visc_law, energy_recon;
-    double ud[kNU];  // gam, 1/Pr, K, K+3, 1/(K+3), 1/dx, dt, dx, R/(gam-1)
+    double ud[kNU];  // gam, 1/Pr, K, K+3, 1/(K+3), 1/dx, dt, dx, R/(gam-1), 1/R, mu0, 1/T0, visc_exp
     float uf[kNU];   // the same rounded to float
 };
 
@@ -127,6 +127,10 @@ __host__ __device__ inline void derive_uniforms(StepPar& p) {
     p.ud[kUDt] = p.dt;
     p.ud[kUDx] = p.dx;
     p.ud[kUCR] = p.R / (p.gam - 1.0);
+    p.ud[kUIR] = 1.0 / p.R;
+    p.ud[kUMu0] = p.mu0;
+    p.ud[kUIT0] = 1.0 / p.T0;
+    p.ud[kUVexp] = p.visc_exp;
     for (int i = 0; i < kNU; i++) p.uf[i] = (float)p.ud[i];
 }
 

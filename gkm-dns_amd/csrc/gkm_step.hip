@@ -20,13 +20,36 @@ namespace gkm {
 // cell, from the FP64 state:
 //   Tw  = (gam-1)(E/rho - |m|^2/(2 rho^2))/R   the WENO temperature input
 //         (update.cpp:560-565)
-//   tau = mu(Ti)/p, Ti = p/(R rho)            the left-cell collision time
-//         (update.cpp:262-264, 882-891)
+//   tau = mu(T)/p, p = rho R T                 the left-cell collision time
+//         (update.cpp:262-264, 882-891: p/(R rho) there is this same T)
 // so that the sweeps read them instead of recomputing them per interface.
+// The temperature recovery is FP64 in both forms, with one reciprocal of rho
+// per cell.  mu(T) and tau are formed in the arithmetic type of the flux that
+// reads them: the power law as exp2(e log2(T/T0)) (FP32: the hardware
+// v_log_f32 / v_exp_f32), Sutherland's law as written.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double viscosity(const StepPar& par, double T) {
-    if (par.visc_law == 1) return par.mu0 * pow(T / par.T0, par.visc_exp);
-    return par.mu0 * (T / par.T0) * sqrt(T / par.T0) * (par.T0 + 110.4) / (T + 110.4);
+template <class Real>
+__device__ __forceinline__ Real collision_time(const StepPar& par, double T, double P) {
+    if (par.visc_law == 1) {
+        const Real x = (Real)(T * par.ud[kUIT0]);
+        Real mu;
+        if constexpr (sizeof(Real) == 8)
+            mu = par.ud[kUMu0] * exp2(par.ud[kUVexp] * log2(x));
+        else
+            mu = par.uf[kUMu0] * __builtin_amdgcn_exp2f(par.uf[kUVexp] * __builtin_amdgcn_logf(x));
+        return mu / (Real)P;
+    }
+    const double x = T * par.ud[kUIT0];
+    return (Real)(par.ud[kUMu0] * x * sqrt(x) * (par.T0 + 110.4) / ((T + 110.4) * P));
+}
+
+template <class Real>
+__device__ __forceinline__ void cell_prims(const StepPar& par, double r0, double m1, double m2, double m3, double E,
+                                           Real& Tw, Real& tau) {
+    const double ri = 1.0 / r0;
+    const double T = (par.gam - 1) * (E * ri - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * (ri * ri)) * par.ud[kUIR];
+    Tw = (Real)T;
+    tau = collision_time<Real>(par, T, r0 * par.R * T);
 }
 
 template <int NPL, class T, class Real, class St>
@@ -52,13 +75,10 @@ __global__ void __launch_bounds__(256) k_ghost_prim(Geo g, Box3 all, Planes10<T>
     }
     if (!Tw) return;
     const StepPar& par = pars[blockIdx.y];
-    const double r0 = st.get(0, src), m1 = st.get(1, src), m2 = st.get(2, src), m3 = st.get(3, src),
-                 E = st.get(4, src);
-    put(Tw, t, (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R));
-    const double u = m1 / r0, v = m2 / r0, w = m3 / r0;
-    const double P = (r0 * (par.gam - 1)) * (E / r0 - 0.5 * (u * u + v * v + w * w));
-    const double Ti = P / (par.R * r0);
-    put(tau, t, (Real)(viscosity(par, Ti) / P));
+    Real Tc, tc;
+    cell_prims<Real>(par, st.get(0, src), st.get(1, src), st.get(2, src), st.get(3, src), st.get(4, src), Tc, tc);
+    put(Tw, t, Tc);
+    put(tau, t, tc);
 }
 
 // ---------------------------------------------------------------------------
@@ -117,12 +137,10 @@ __global__ void __launch_bounds__(256) k_prim(Geo g, St st, Real* Tw, Real* tau,
     if (tl >= g.ncell) return;
     const unsigned t = blockIdx.y * g.ncell + tl;
     const StepPar& par = pars[blockIdx.y];
-    const double r0 = st.get(0, t), m1 = st.get(1, t), m2 = st.get(2, t), m3 = st.get(3, t), E = st.get(4, t);
-    put(Tw, t, (Real)((par.gam - 1) * (E / r0 - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / (r0 * r0)) / par.R));
-    const double u = m1 / r0, v = m2 / r0, w = m3 / r0;
-    const double P = (r0 * (par.gam - 1)) * (E / r0 - 0.5 * (u * u + v * v + w * w));
-    const double Ti = P / (par.R * r0);
-    put(tau, t, (Real)(viscosity(par, Ti) / P));
+    Real Tc, tc;
+    cell_prims<Real>(par, st.get(0, t), st.get(1, t), st.get(2, t), st.get(3, t), st.get(4, t), Tc, tc);
+    put(Tw, t, Tc);
+    put(tau, t, tc);
 }
 
 // ---------------------------------------------------------------------------
@@ -387,9 +405,9 @@ __global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigne
     }
     const double r = w0[0], u = w0[1] / r, v = w0[2] / r, w = w0[3] / r;
     const double P = (r * (par.gam - 1)) * (w0[4] / r - 0.5 * (u * u + v * v + w * w));
-    const double tau = viscosity(par, P / (par.R * r)) / P;
+    const Real tau = collision_time<Real>(par, P / (par.R * r), P);
     Real Fs[5];
-    gks_flux(gas, W0, Dx, Dy, Dz, (Real)tau, upar<Real>(par, kUDt), Fs);
+    gks_flux(gas, W0, Dx, Dy, Dz, tau, upar<Real>(par, kUDt), Fs);
 #pragma unroll
     for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
