@@ -421,21 +421,21 @@ __device__ __forceinline__ void solve_frame(const Gas<T>& g, const T b[5], T U, 
 // tangential slopes), B (source of its time slope) and F (the flux).
 template <class T>
 __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, const NTab<T>& nh, const T DWx[5],
-                                           const T DWy[5], const T DWz[5], T c45, T c5, T p3, T p4, T p5, T by[5],
-                                           T bz[5], T B[5], T F[5]) {
+                                           const T DWy[5], const T DWz[5], T sx, T st, T c45, T c5, T p3, T p4,
+                                           T p5, T by[5], T bz[5], T B[5], T F[5]) {
     const T h = s.h;
-    const T ri = s.ri;
+    const T rx = sx * s.ri, rt = st * s.ri;  // slope scale / rho
     T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWx[q] * ri;
+    for (int q = 0; q < 5; q++) b[q] = DWx[q] * rx;
     to_frame(b, s.V, s.W, bp);
     solve_frame(g, bp, s.U, s.lam, h, ax);
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWy[q] * ri;
+    for (int q = 0; q < 5; q++) b[q] = DWy[q] * rt;
     to_frame(b, s.V, s.W, bp);
     solve_frame(g, bp, s.U, s.lam, h, ay);
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWz[q] * ri;
+    for (int q = 0; q < 5; q++) b[q] = DWz[q] * rt;
     to_frame(b, s.V, s.W, bp);
     solve_frame(g, bp, s.U, s.lam, h, az);
 #pragma unroll
@@ -495,10 +495,12 @@ __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, co
 // All vectors are in the rotated slot order (slot 1 = normal momentum).
 // tau0 = mu/p of the left cell.  slopes(side, Dx, Dy, Dz) yields the slopes
 // left (side 0) or right (side 1) of the interface along the normal (x) and
-// the two tangential (y, z) directions.
+// the two tangential (y, z) directions, divided by sx (normal) and st
+// (tangential): the scale factors are applied together with 1/rho.
 template <class T, class LoadSlopes>
 __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T WR[5], const T Wl[5],
-                                         const T Wr[5], LoadSlopes&& slopes, T tau0, T dt, T dx, T F[5]) {
+                                         const T Wr[5], LoadSlopes&& slopes, T sx, T st, T tau0, T dt, T dx,
+                                         T F[5]) {
     Maxw<T> l, r;
     maxw_from_W(g, Wl, l);
     maxw_from_W(g, Wr, r);
@@ -553,12 +555,12 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     {
         T Dx[5], Dy[5], Dz[5];
         slopes(0, Dx, Dy, Dz);  // left side
-        side_block(g, l, hl, Dx, Dy, Dz, c45, gm5, p3, p4, p5, by, bz, B, Fa);
+        side_block(g, l, hl, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
     }
     {
         T Dx[5], Dy[5], Dz[5];
         slopes(1, Dx, Dy, Dz);  // right side
-        side_block(g, r, hr, Dx, Dy, Dz, c45, gm5, p3, p4, p5, by, bz, B, Fa);
+        side_block(g, r, hr, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
     }
 
     // equilibrium state, in its own co-moving frame

@@ -322,6 +322,7 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
     gas.kp3 = upar<Real>(par, kUKp3);
     gas.ikp3 = upar<Real>(par, kUIkp3);
     const Real idx = upar<Real>(par, kUIdx);
+    const Real idx2 = 2 * idx, hidx = Real(0.5) * idx;
 
     // collision time mu/p of the left cell (update.cpp:262-264), from the plane
     const Real tau0 = ld(tau, c);
@@ -339,12 +340,14 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
         for (int s = 0; s < 5; s++) {
             const int q = slot_field<DIR>(s);
             const int o = side == 0 ? q : 5 + q;
-            Dx[s] = side == 0 ? 2 * (Wl[s] - WL[s]) * idx : 2 * (WR[s] - Wr[s]) * idx;
-            Dy[s] = Real(0.5) * (ld(wi.p[o], c + st1) - ld(wi.p[o], c - st1)) * idx;
-            Dz[s] = Real(0.5) * (ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2)) * idx;
+            // differences; the flux scales them by 2/dx (normal: half-cell
+            // distance) and 1/(2 dx) (tangential: central difference)
+            Dx[s] = side == 0 ? Wl[s] - WL[s] : WR[s] - Wr[s];
+            Dy[s] = ld(wi.p[o], c + st1) - ld(wi.p[o], c - st1);
+            Dz[s] = ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2);
         }
     };
-    bgk_flux(gas, WL, WR, Wl, Wr, slopes, tau0, upar<Real>(par, kUDt), upar<Real>(par, kUDx), Fs);
+    bgk_flux(gas, WL, WR, Wl, Wr, slopes, idx2, hidx, tau0, upar<Real>(par, kUDt), upar<Real>(par, kUDx), Fs);
 }
 
 template <class Real, class St, int DIR>
@@ -615,7 +618,7 @@ __global__ void k_flux_points(int n, const double* rec, double dx, double gam, d
         }
     };
     Real F[5];
-    bgk_flux(gas, v[0], v[1], v[2], v[3], slopes, (Real)x[50], (Real)x[51], (Real)dx, F);
+    bgk_flux(gas, v[0], v[1], v[2], v[3], slopes, Real(1), Real(1), (Real)x[50], (Real)x[51], (Real)dx, F);
     for (int q = 0; q < 5; q++) out[(long)t * 5 + q] = (double)F[q];
 }
 
