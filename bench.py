@@ -16,7 +16,9 @@ dilatational dissipation and the velocity-derivative skewness over the run.
 
 The CPU baseline is the FP64 oracle (the restated reference CPU step) on a
 bounded sample: a 64^3 box of the same recipe for a few steps, single-threaded
-like the reference, plus an OpenMP run over the host cores.
+like the reference, plus an OpenMP run over the host cores.  The same leg checks
+the 128^3 FP64 drop-in case (field "dropin_128": Mt 0.3, Re 50, 500 timed
+steps) against the oracle after 10 steps.
 """
 from __future__ import annotations
 
@@ -73,6 +75,8 @@ def parse():
     ap.add_argument("--no-long", action="store_true", help="skip the mixed-vs-FP64 decay run to t/t0 = 4")
     ap.add_argument("--long-t", type=float, default=4.0, help="end of the decay run in eddy-turnover times")
     ap.add_argument("--long-every", type=int, default=25, help="steps between statistics samples")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the 128^3 FP64 drop-in case")
+    ap.add_argument("--dropin-steps", type=int, default=500)
     ap.add_argument("--slab-steps", type=int, default=3)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=64)
@@ -93,8 +97,59 @@ def time_solver(s, dt, steps, warmup):
     return (t1 - t0), tim
 
 
-def cpu_baseline(args, gas_kw):
-    """FP64 oracle on a bounded sample of the workload (same recipe, smaller box)."""
+def bench_dropin_128(args, kw):
+    """The 128^3 FP64 drop-in case (k0 = 4, Mt = 0.3, Re = 50, gamma = 1.4,
+    Pr = 0.7): the state after 10 steps is kept for the oracle comparison in the
+    CPU leg, then 500 further steps are timed."""
+    import gkm_amd
+
+    ic = gkm_amd.dhit(128, k0=4, Mt=0.3, Re=50, seed=1234)
+    s = gkm_amd.Solver.from_ic(ic, precision="fp64", **kw)
+    dt = s.timestep(args.cfl)
+    W0 = s.get_state()
+    s.evolve_eulerian(10, dt=dt)
+    W10 = s.get_state()
+    s.evolve_eulerian(3, dt=dt)
+    s.sync()
+    t0 = time.perf_counter()
+    s.evolve_eulerian(args.dropin_steps, dt=dt)
+    s.sync()
+    sec = time.perf_counter() - t0
+    gas = s.gas
+    s.close()
+    print(f"[bench] 128^3 fp64: {1e3 * sec / args.dropin_steps:.3f} ms/step", file=sys.stderr, flush=True)
+    res = {"box": [128, 128, 128], "precision": "fp64", "steps": args.dropin_steps,
+           "value": 128.0 ** 3 * args.dropin_steps / sec, "unit": "cell-updates/s",
+           "ms_per_step": 1e3 * sec / args.dropin_steps, "rel_l2_vs_oracle_10_steps": None}
+    return res, (W0, W10, dt, gas)
+
+
+def dropin_parity(state, threads):
+    """Relative L2 of the device's 10-step 128^3 state against the FP64 oracle."""
+    import numpy as np
+
+    import gkm_amd._abi as A
+    from oracle import oracle as O
+
+    W0, W10, dt, gas = state
+    g = O.make_grid(128, 128, 128, 2 * math.pi / 128)
+    P = O.make_params(gamma=gas.gamma, R=gas.R, Pr=gas.Pr, mu0=gas.mu0, T0=gas.T0, visc_exp=gas.visc_exp,
+                      K=gas.dof(), visc_law=A.VISC[gas.visc_law], recon=A.RECON[gas.recon],
+                      energy_recon=A.ENERGY[gas.energy_recon], zbug_compat=int(gas.zbug_compat))
+    Wo = [w.copy() for w in W0]
+    t0 = time.perf_counter()
+    O.step(g, P, Wo, dt, 10, threads)
+    a = np.concatenate([np.ravel(x) for x in W10])
+    b = np.concatenate([np.ravel(x) for x in Wo])
+    err = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print(f"[bench] 128^3 oracle, {threads} threads, 10 steps in {time.perf_counter() - t0:.1f} s: rel L2 {err:.2e}",
+          file=sys.stderr, flush=True)
+    return err
+
+
+def cpu_baseline(args, gas_kw, dropin_state=None):
+    """FP64 oracle on a bounded sample of the workload (same recipe, smaller box);
+    with dropin_state, also the oracle check of the 128^3 drop-in case."""
     import numpy as np  # noqa: F401
 
     from gkm_amd.ic import conservative_padded, dhit
@@ -128,9 +183,12 @@ def cpu_baseline(args, gas_kw):
                 break
     except OSError:
         pass
-    return {"value": one, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle, {n}^3 box of the same recipe, {args.cpu_steps} steps, 1 thread",
-            "omp_value": many, "omp_threads": threads, "cpu_model": cpu_model}
+    res = {"value": one, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+           "sample": f"FP64 oracle, {n}^3 box of the same recipe, {args.cpu_steps} steps, 1 thread",
+           "omp_value": many, "omp_threads": threads, "cpu_model": cpu_model}
+    if dropin_state is not None:
+        res["dropin_128_rel_l2"] = dropin_parity(dropin_state, threads)
+    return res
 
 
 def bench_gks(args, ic, dt):
@@ -367,12 +425,17 @@ def main():
         out["gks"] = bench_gks(args, ic, dt)
     if not args.no_fp64 and not args.no_long:
         out["long_run"] = bench_long_run(args, ic, kw)
+    dropin_state = None
+    if not args.no_dropin:
+        out["dropin_128"], dropin_state = bench_dropin_128(args, kw)
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args)
     if not args.no_slab:
         out["slab"] = bench_slab(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, kw)
+        out["cpu_baseline"] = cpu_baseline(args, kw, dropin_state)
+        if dropin_state is not None:
+            out["dropin_128"]["rel_l2_vs_oracle_10_steps"] = out["cpu_baseline"].pop("dropin_128_rel_l2")
     print(json.dumps(out))
 
 

@@ -146,6 +146,26 @@ def test_step_parity_dhit32(gk, orc):
     assert err <= FP64_TOL
 
 
+def test_step_parity_dhit128(gk, orc):
+    """The 128^3 drop-in case: decaying-turbulence initial condition (k0 = 4,
+    Mt = 0.3, Re = 50, gamma = 1.4, Pr = 0.7, power-law viscosity, WENO-JS),
+    10 FP64 steps at CFL 0.5 against the oracle, relative L2 <= 1e-12."""
+    s = gk.Solver.from_dhit(128, k0=4, Mt=0.3, Re=50, seed=1234, visc_law="power", recon="js", Pr=0.7)
+    dt = s.timestep(0.5)
+    W0 = s.get_state()
+    s.evolve_eulerian(10)
+    Wd = s.get_state()
+    gas = s.gas
+    s.close()
+    print("dhit128: device done, oracle on", NTHREADS, "threads", flush=True)
+    Wo = [w.copy() for w in W0]
+    grid = orc.make_grid(128, 128, 128, 2 * math.pi / 128)
+    orc.step(grid, _orc_params(orc, gas), Wo, dt, 10, NTHREADS)
+    err = rel_l2(Wd, Wo)
+    print("dhit128 rel L2", err)
+    assert err <= FP64_TOL
+
+
 def test_ghost_fill_bitwise(gk, orc):
     nx, ny, nz = 9, 7, 11
     W = [np.random.default_rng(q).random((ny + 6) * (nx + 6) * (nz + 6)) for q in range(5)]
