@@ -109,3 +109,70 @@ def test_dhit_uneven_box():
     ic = dhit(12, ny=8, nz=16, seed=2)
     assert ic.u.shape == (8, 12, 16)
     assert np.all(np.isfinite(ic.u))
+
+
+def _create(**changes):
+    import gkm_amd._abi as A
+    from gkm_amd.solver import GasParams
+    L = A.load()
+    grid = A.GkmGrid(8, 8, 8, 3, 0.1)
+    p = GasParams().to_c()
+    for k, v in changes.items():
+        if hasattr(grid, k):
+            setattr(grid, k, v)
+        else:
+            setattr(p, k, v)
+    h = C.c_void_p()
+    rc = L.gkm_create(C.byref(grid), C.byref(p), C.byref(h))
+    return rc, L.gkm_last_error().decode(), h
+
+
+@pytest.mark.parametrize("changes,word", [
+    (dict(nx=2), "too small"), (dict(nz=0), "too small"), (dict(dx=0.0), "dx"), (dict(dx=float("nan")), "dx"),
+    (dict(gamma=1.0), "gamma"), (dict(Pr=0.0), "Pr"), (dict(mu0=-1.0), "mu0"), (dict(T0=0.0), "T0"),
+    (dict(K=-1), "K"), (dict(visc_law=7), "visc_law"), (dict(recon=5), "recon"),
+    (dict(energy_recon=-1), "energy_recon"), (dict(precision=3), "precision"), (dict(flux_scheme=9), "flux_scheme"),
+])
+def test_parameter_checks_run_before_any_device_call(changes, word):
+    """gkm_create validates the grid and the gas parameters on the host
+    (GKM_E_ARG with a message naming the field) and leaves *out null."""
+    import gkm_amd._abi as A
+    rc, msg, h = _create(**changes)
+    assert rc == A.GKM_E_ARG, (changes, rc, msg)
+    assert word in msg, msg
+    assert not h.value
+
+
+def test_batch_checks_run_before_any_device_call():
+    import gkm_amd._abi as A
+    from gkm_amd.solver import GasParams
+    L = A.load()
+    grid = A.GkmGrid(8, 8, 8, 3, 0.1)
+    ps = (A.GkmParams * 2)(GasParams().to_c(), GasParams(recon="js").to_c())
+    h = C.c_void_p()
+    assert L.gkm_create_batch(C.byref(grid), ps, 2, C.byref(h)) == A.GKM_E_ARG
+    assert b"must match box 0" in L.gkm_last_error()
+    assert L.gkm_create_batch(C.byref(grid), ps, 0, C.byref(h)) == A.GKM_E_ARG
+    big = A.GkmGrid(1024, 1024, 1024, 3, 0.1)  # 1030^3 padded cells > 2^28
+    assert L.gkm_create_batch(C.byref(big), ps, 1, C.byref(h)) == A.GKM_E_ARG
+    assert b"2^28" in L.gkm_last_error()
+    assert not h.value
+
+
+def test_null_handles_are_rejected():
+    """Every entry point taking a handle returns GKM_E_ARG for a null one
+    (gkm_destroy(NULL) is a no-op, as free(NULL))."""
+    import gkm_amd._abi as A
+    L = A.load()
+    d = C.c_double()
+    n = C.c_int()
+    st = A.GkmStats()
+    assert L.gkm_destroy(None) == A.GKM_OK
+    assert L.gkm_step(None, C.c_double(1e-3), 1) == A.GKM_E_ARG
+    assert L.gkm_compute_dt(None, C.c_double(0.5), C.byref(d)) == A.GKM_E_ARG
+    assert L.gkm_stats(None, C.byref(st)) == A.GKM_E_ARG
+    assert L.gkm_fill_ghosts(None) == A.GKM_E_ARG
+    assert L.gkm_sync(None) == A.GKM_E_ARG
+    assert L.gkm_batch_size(None, C.byref(n)) == A.GKM_E_ARG
+    assert L.gkm_set_graph(None, 1) == A.GKM_E_ARG
+    assert L.gkm_set_timing(None, 1) == A.GKM_E_ARG
