@@ -79,9 +79,11 @@ void drop_graph(Ctx* h) {
 int upload_pars(Ctx* h, const double* dt) {
     bool same = h->pars_valid;
     for (int b = 0; b < h->nbox; b++) {
+        // step_par zero-fills the struct, padding included, and the bytes are
+        // copied as they are, so the comparison sees only the values
         gkm::StepPar p = gkm::step_par(h->grid, h->bpar[b], dt ? dt[b] : 0.0);
         if (!same || std::memcmp(&p, &h->h_pars[b], sizeof p) != 0) same = false;
-        h->h_pars[b] = p;
+        std::memcpy(&h->h_pars[b], &p, sizeof p);
     }
     if (same) return GKM_OK;
     HIP_TRY(hipMemcpyAsync(h->d_pars, h->h_pars, sizeof(gkm::StepPar) * h->nbox, hipMemcpyHostToDevice, h->stream));
@@ -216,6 +218,37 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
 
 int gkm_create(const gkm_grid* grid, const gkm_params* params, gkm_t** out) {
     return gkm_create_batch(grid, params, 1, out);
+}
+
+int gkm_get_grid(gkm_t* hh, gkm_grid* out) {
+    Ctx* h = ctx(hh);
+    if (!h || !out) return fail(GKM_E_ARG, "null argument");
+    *out = h->grid;
+    return GKM_OK;
+}
+
+int gkm_get_params_box(gkm_t* hh, int box, gkm_params* out) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!out) return fail(GKM_E_ARG, "null output");
+    *out = h->bpar[box];
+    return GKM_OK;
+}
+
+int gkm_set_params_box(gkm_t* hh, int box, const gkm_params* p) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    rc = check_params(&h->grid, p);
+    if (rc) return rc;
+    if (p->recon != h->par.recon || p->energy_recon != h->par.energy_recon || p->zbug_compat != h->par.zbug_compat ||
+        p->precision != h->par.precision || p->flux_scheme != h->par.flux_scheme)
+        return fail(GKM_E_ARG, "recon, energy_recon, zbug_compat, precision and flux_scheme must match the handle's");
+    h->bpar[box] = *p;
+    if (box == 0) h->par = *p;
+    h->pars_valid = false;
+    return GKM_OK;
 }
 
 int gkm_batch_size(gkm_t* hh, int* nbox) {

@@ -10,7 +10,14 @@
 //   PrintStats                final line
 //   WriteReInit (optional)    restart files of the final state
 //
-// usage: gkm_dns <case-dir> [--proc-dim a b c] [--tmax T | --steps N]
+// With --dhit N instead of a case folder, the library generates the
+// init_BS.m initial condition itself (gkm_init_dhit: N^3, --k0, --mt, --re,
+// --seed, --gamma, density 1), and --write-case DIR writes it out as a case
+// folder (params + Init rank files) that the reference can read.
+//
+// usage: gkm_dns <case-dir> | --dhit N [--k0 k] [--mt M] [--re R] [--seed s]
+//        [--gamma g] [--write-case DIR]
+//        [--proc-dim a b c] [--tmax T | --steps N]
 //        [--cfl C] [--precision fp64|mixed] [--recon symoo|js]
 //        [--visc sutherland|power] [--visc-exp e] [--energy T|E] [--pr Pr]
 //        [--R R] [--z-fixed] [--scheme weno|gks] [--reinit DIR] [--timing]
@@ -24,6 +31,7 @@
 #include <vector>
 
 #include "../../include/gkm.h"
+#include "../../include/gkm_ic.h"
 #include "../../include/gkm_io.h"
 
 namespace {
@@ -41,7 +49,8 @@ void die(const char* what, int rc) {
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: gkm_dns <case-dir> [--proc-dim a b c] [--tmax T | --steps N] [--cfl C]\n"
+                 "usage: gkm_dns <case-dir> | --dhit N [--k0 k] [--mt M] [--re R] [--seed s] [--gamma g]\n"
+                 "               [--write-case DIR] [--proc-dim a b c] [--tmax T | --steps N] [--cfl C]\n"
                  "               [--precision fp64|mixed] [--recon symoo|js] [--visc sutherland|power]\n"
                  "               [--visc-exp e] [--energy T|E] [--pr Pr] [--R R] [--z-fixed]\n"
                  "               [--scheme weno|gks] [--reinit DIR] [--timing]\n");
@@ -66,8 +75,11 @@ int main(int argc, char** argv) {
     par.energy_recon = GKM_ENERGY_T;
     par.zbug_compat = 1;
     par.precision = GKM_PREC_FP64;
-    std::string reinit;
+    std::string reinit, write_case;
     bool timing = false;
+    int dhit_n = 0;
+    gkm_ic ic = {1234ull, 4.0, 0.3, 30.0, 1.0};
+    double dhit_gamma = 1.4;
     for (int a = 1; a < argc; a++) {
         const std::string s = argv[a];
         auto next = [&]() -> const char* {
@@ -111,23 +123,58 @@ int main(int argc, char** argv) {
             reinit = next();
         } else if (s == "--timing") {
             timing = true;
+        } else if (s == "--dhit") {
+            dhit_n = std::atoi(next());
+        } else if (s == "--k0") {
+            ic.k0 = std::atof(next());
+        } else if (s == "--mt") {
+            ic.Mt = std::atof(next());
+        } else if (s == "--re") {
+            ic.Re = std::atof(next());
+        } else if (s == "--seed") {
+            ic.seed = std::strtoull(next(), nullptr, 10);
+        } else if (s == "--gamma") {
+            dhit_gamma = std::atof(next());
+        } else if (s == "--write-case") {
+            write_case = next();
         } else if (!s.empty() && s[0] == '-') {
             usage();
         } else {
             dir = s;
         }
     }
-    if (dir.empty()) usage();
-
-    // GetParams (weno/src/FlowField.cpp:9-57)
-    gkm_weno_params wp;
-    CALL(gkm_read_weno_params((dir + "/params").c_str(), &wp));
+    if (dir.empty() == (dhit_n == 0)) usage();  // a case folder or --dhit, not both
     const double pi = std::atan(1.0) * 4.0;
+    gkm_weno_params wp;
+    gkm_t* h = nullptr;
+    if (dhit_n > 0) {
+        // GetParams + Initialize_DHIT from the library's generator
+        wp.nc = dhit_n;
+        wp.gam = dhit_gamma;
+        par.gamma = dhit_gamma;
+        par.K = (int)((5.0 - 3.0 * dhit_gamma) / (dhit_gamma - 1.0));
+        par.mu0 = 0.0;  // set from the recipe by gkm_init_dhit
+        par.T0 = 1.0;
+        gkm_grid g0 = {dhit_n, dhit_n, dhit_n, 3, 2.0 * pi / dhit_n};
+        CALL(gkm_create(&g0, &par, &h));
+        gkm_ic_info info;
+        CALL(gkm_init_dhit(h, &ic, &info));
+        CALL(gkm_get_params_box(h, 0, &par));
+        wp.den0 = ic.rho0;
+        wp.Mt = ic.Mt;
+        wp.Re = ic.Re;
+        wp.mu0 = info.mu0;
+        wp.p0 = info.p0;
+        wp.t0 = info.t0;
+    } else {
+        // GetParams (weno/src/FlowField.cpp:9-57)
+        CALL(gkm_read_weno_params((dir + "/params").c_str(), &wp));
+        par.gamma = wp.gam;
+        par.K = (int)((5.0 - 3.0 * wp.gam) / (wp.gam - 1.0));
+        par.mu0 = wp.mu0;
+        par.T0 = wp.p0 / (par.R * wp.den0);
+    }
     const int nc = wp.nc;
-    par.gamma = wp.gam;
-    par.K = (int)((5.0 - 3.0 * wp.gam) / (wp.gam - 1.0));
-    par.mu0 = wp.mu0;
-    par.T0 = wp.p0 / (par.R * wp.den0);
     gkm_grid grid = {nc, nc, nc, 3, 2.0 * pi / nc};
     if (tmax < 0 && steps < 0) tmax = 7.0 * wp.t0;  // tmax = 7 t0 (param.h:62)
 
@@ -149,11 +196,26 @@ int main(int argc, char** argv) {
     std::vector<double> store(5 * ncell, 0.0);
     double* W[5];
     for (int q = 0; q < 5; q++) W[q] = store.data() + q * ncell;
-    CALL(gkm_read_weno_init(dir.c_str(), nc, pd, wp.gam, W));
-
-    gkm_t* h = nullptr;
-    CALL(gkm_create(&grid, &par, &h));
-    CALL(gkm_set_state(h, W));
+    auto make_dir = [](const std::string& d) {
+        std::error_code ec;
+        std::filesystem::create_directories(d, ec);
+        if (ec) {
+            std::fprintf(stderr, "gkm_dns: cannot create %s: %s\n", d.c_str(), ec.message().c_str());
+            std::exit(1);
+        }
+    };
+    if (dhit_n > 0) {
+        if (!write_case.empty()) {  // the case folder init_BS.m would have written
+            make_dir(write_case);
+            CALL(gkm_write_weno_params((write_case + "/params").c_str(), &wp));
+            CALL(gkm_get_state(h, W));
+            CALL(gkm_write_weno_init(write_case.c_str(), nc, pd, wp.gam, W));
+        }
+    } else {
+        CALL(gkm_read_weno_init(dir.c_str(), nc, pd, wp.gam, W));
+        CALL(gkm_create(&grid, &par, &h));
+        CALL(gkm_set_state(h, W));
+    }
     CALL(gkm_fill_ghosts(h));  // HaloTransfer before the loop (update.cpp:106-110)
     double dt = 0.0;
     CALL(gkm_compute_dt(h, cfl, &dt));  // timestep(W), once (update.cpp:112)
@@ -185,12 +247,7 @@ int main(int argc, char** argv) {
                      iter ? (double)nc * nc * nc * iter / (step_ms * 1e-3) : 0.0);
 
     if (!reinit.empty()) {  // WriteReInit (update.cpp:1352-1391)
-        std::error_code ec;
-        std::filesystem::create_directories(reinit, ec);
-        if (ec) {
-            std::fprintf(stderr, "gkm_dns: cannot create %s: %s\n", reinit.c_str(), ec.message().c_str());
-            return 1;
-        }
+        make_dir(reinit);
         CALL(gkm_get_state(h, W));
         CALL(gkm_write_weno_init(reinit.c_str(), nc, pd, wp.gam, W));
     }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <utility>
 
 #include "gkm_internal.h"
@@ -899,6 +900,7 @@ static void launch_prim(Ctx* h) {
 
 StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
     StepPar p;
+    std::memset(&p, 0, sizeof p);
     p.gam = pp.gamma;
     p.R = pp.R;
     p.invPr = 1.0 / pp.Pr;

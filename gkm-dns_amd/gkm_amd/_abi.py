@@ -35,7 +35,9 @@ EXPORTS = (
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
     "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats", "gkm_gks_points",
-    "gkm_last_sweep_timing",
+    "gkm_last_sweep_timing", "gkm_get_grid", "gkm_get_params_box", "gkm_set_params_box",
+    # include/gkm_ic.h
+    "gkm_dhit_velocity", "gkm_init_dhit_box", "gkm_init_dhit",
     # include/gkm_io.h
     "gkm_read_weno_params", "gkm_write_weno_params", "gkm_read_weno_init", "gkm_write_weno_init",
     "gkm_read_init_dat", "gkm_write_init_dat", "gkm_init_dat_state",
@@ -55,6 +57,21 @@ class GkmParams(C.Structure):
         ("recon", C.c_int), ("energy_recon", C.c_int), ("zbug_compat", C.c_int), ("precision", C.c_int),
         ("flux_scheme", C.c_int),
     ]
+
+
+class GkmIc(C.Structure):
+    _fields_ = [("seed", C.c_ulonglong), ("k0", C.c_double), ("Mt", C.c_double), ("Re", C.c_double),
+                ("rho0", C.c_double)]
+
+
+IC_INFO_FIELDS = ("q2", "urms", "c0", "p0", "mu0", "t0")
+
+
+class GkmIcInfo(C.Structure):
+    _fields_ = [(f, C.c_double) for f in IC_INFO_FIELDS]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in IC_INFO_FIELDS}
 
 
 STAT_FIELDS = (

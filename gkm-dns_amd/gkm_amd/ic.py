@@ -108,6 +108,30 @@ def dhit(n: int, k0: float = 4.0, Mt: float = 0.3, Re: float = 30.0, seed: int =
                   gamma=gamma)
 
 
+def dhit_native(n: int, k0: float = 4.0, Mt: float = 0.3, Re: float = 30.0, seed: int = 1234, rho0: float = 1.0,
+                gamma: float = 1.4, ny: int | None = None, nz: int | None = None) -> DhitIC:
+    """The same recipe generated by the library (gkm_dhit_velocity, host code,
+    SplitMix64 random numbers): the field differs from dhit()'s for a given
+    seed, the spectrum and the derived scales do not."""
+    import ctypes as C
+
+    from . import _abi
+
+    nx = n
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    u = np.empty((ny, nx, nz))
+    v = np.empty((ny, nx, nz))
+    w = np.empty((ny, nx, nz))
+    icp = _abi.GkmIc(int(seed) & 0xFFFFFFFFFFFFFFFF, float(k0), float(Mt), float(Re), float(rho0))
+    info = _abi.GkmIcInfo()
+    dp = C.POINTER(C.c_double)
+    _abi.check(_abi.load().gkm_dhit_velocity(nx, ny, nz, C.byref(icp), C.c_double(gamma), u.ctypes.data_as(dp),
+                                             v.ctypes.data_as(dp), w.ctypes.data_as(dp), C.byref(info)))
+    return DhitIC(u=u, v=v, w=w, rho0=rho0, p0=info.p0, mu0=info.mu0, t0=info.t0, urms=info.urms, c0=info.c0, Mt=Mt,
+                  Re=Re, k0=k0, gamma=gamma)
+
+
 def conservative_padded(ic: DhitIC, ghost: int = 3):
     """Padded conservative planes W0..W4 (ghosts zero) from uniform rho0, p0 and
     the IC velocity, as weno/src/FlowField.cpp:97-102 forms them."""

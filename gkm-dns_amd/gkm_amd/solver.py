@@ -108,6 +108,23 @@ class Solver:
         s.initialize_dhit(ic)
         return s
 
+    def init_dhit(self, k0: float = 4.0, Mt: float = 0.3, Re: float = 30.0, seed: int = 1234,
+                  rho0: float = 1.0) -> dict:
+        """GetParams + Initialize_DHIT in the library (gkm_init_dhit): the
+        init_BS.m recipe generated by libgkm on the host, written into the box,
+        ghosts filled, and the box's mu0 and T0 set from it.  Returns the
+        derived scales (q2, urms, c0, p0, mu0, t0)."""
+        icp = _abi.GkmIc(int(seed) & 0xFFFFFFFFFFFFFFFF, float(k0), float(Mt), float(Re), float(rho0))
+        info = _abi.GkmIcInfo()
+        _abi.check(self._L.gkm_init_dhit(self._h, C.byref(icp), C.byref(info)))
+        d = info.as_dict()
+        self.gas.mu0 = d["mu0"]
+        self.gas.T0 = d["p0"] / (self.gas.R * rho0)
+        self._params = self.gas.to_c()
+        self.info.t0 = d["t0"]
+        self.norm = {"Mt": Mt, "p0": d["p0"], "rho0": rho0}
+        return d
+
     def initialize_dhit(self, ic: DhitIC) -> None:
         """Initialize_DHIT (FlowField.cpp:59-112) followed by the first halo fill."""
         self.ic = ic

@@ -158,9 +158,21 @@ int gkm_step_batch(gkm_t* h, const double* dt, int nsteps);
 int gkm_compute_dt_box(gkm_t* h, int box, double cfl, double* dt);
 int gkm_stats_box(gkm_t* h, int box, gkm_stats_t* out);
 
+/* The handle's grid, and box `box`'s gas parameters. */
+int gkm_get_grid(gkm_t* h, gkm_grid* out);
+int gkm_get_params_box(gkm_t* h, int box, gkm_params* out);
+/* Replace box `box`'s gas constants (gamma, R, Pr, mu0, T0, visc_exp, K,
+ * visc_law); recon, energy_recon, zbug_compat, precision and flux_scheme must
+ * equal the handle's.  Takes effect from the next step (also in a captured
+ * graph, which reads the constants from the device table).  No reference
+ * counterpart: the reference's constants are globals set once by GetParams. */
+int gkm_set_params_box(gkm_t* h, int box, const gkm_params* p);
+
 /* Capture one step (ghost fill, three sweeps, update) in a hipGraph on first
- * use and replay it for every later step; recaptured when dt changes.  Off by
- * default.  Steps timed with gkm_set_timing are launched without the graph. */
+ * use and replay it for every later step.  dt and the gas constants are read
+ * from a device table at replay, so changing them needs no new capture.  Off
+ * by default.  Steps timed with gkm_set_timing are launched without the
+ * graph. */
 int gkm_set_graph(gkm_t* h, int enabled);
 
 /* Launch shapes: threads per block of the flux kernels (64, 128 or 256) and
