@@ -260,6 +260,10 @@ def bench_ensemble(args):
     ek = np.array([abs(a["ek"] - b["ek"]) / abs(b["ek"]) for a, b in zip(sm, sd)])
     mt = np.array([abs(a["mt"] - b["mt"]) / abs(b["mt"]) for a, b in zip(sm, sd)])
     cells = em.cells
+    # the same ensemble advanced by one graph launch per step (gkm_group_step)
+    em.use_group(True)
+    em.step(2)
+    sec_g = em.run_timed(args.ensemble_steps)
     em.close()
     ed.close()
     print(f"[bench] ensemble: {1e3 * sec / args.ensemble_steps:.2f} ms/step mixed, "
@@ -268,6 +272,7 @@ def bench_ensemble(args):
             "value": cells * args.ensemble_steps / sec, "unit": "cell-updates/s",
             "ms_per_step": 1e3 * sec / args.ensemble_steps,
             "fp64_value": cells * args.ensemble_steps / sec_d,
+            "one_graph_launch_value": cells * args.ensemble_steps / sec_g,
             "ek_rel_err_max": float(ek.max()), "ek_rel_err_median": float(np.median(ek)),
             "mt_rel_err_max": float(mt.max()),
             "tuning": {str(n): list(v) for n, v in tuning.items()}, "tune_seconds": t_tune,

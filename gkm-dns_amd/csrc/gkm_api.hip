@@ -370,6 +370,14 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
     return GKM_OK;
 }
 
+int gkm_set_dt_batch(gkm_t* hh, const double* dt) {
+    Ctx* h = ctx(hh);
+    if (!h || !dt) return fail(GKM_E_ARG, "null handle or dt");
+    for (int b = 0; b < h->nbox; b++)
+        if (!(dt[b] > 0) || !std::isfinite(dt[b])) return fail(GKM_E_ARG, "dt[%d] must be positive and finite", b);
+    return upload_pars(h, dt);
+}
+
 int gkm_step(gkm_t* hh, double dt, int nsteps) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
@@ -404,6 +412,155 @@ int gkm_sync(gkm_t* hh) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return GKM_OK;
+}
+
+// ---- groups ----------------------------------------------------------------
+// One graph per combination of the handles' current state buffers (the
+// WENO-BGK step alternates them; all handles of a group step together, so at
+// most two combinations occur).
+struct gkm_group {
+    std::vector<Ctx*> hs;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> pre, join;  // per handle: before the group, end of its branch
+    struct Cap {
+        std::vector<int> flips;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Cap> caps;
+};
+
+static void free_group(gkm_group* g) {
+    for (auto& c : g->caps) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+    }
+    for (auto e : g->pre)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : g->join)
+        if (e) (void)hipEventDestroy(e);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->done) (void)hipEventDestroy(g->done);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+int gkm_group_create(gkm_t* const* handles, int n, gkm_group_t** out) {
+    if (!out) return fail(GKM_E_ARG, "null output");
+    *out = nullptr;
+    if (!handles || n < 1) return fail(GKM_E_ARG, "need n >= 1 handles");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    gkm_group* g = new (std::nothrow) gkm_group();
+    if (!g) return fail(GKM_E_NOMEM, "host allocation failed");
+    g->device = dev;
+    for (int i = 0; i < n; i++) {
+        Ctx* h = ctx(handles[i]);
+        bool dup = false;
+        for (Ctx* o : g->hs) dup |= o == h;
+        if (!h || dup || h->device != dev) {
+            delete g;
+            return fail(GKM_E_ARG, "handle %d is null, repeated or on another device", i);
+        }
+        g->hs.push_back(h);
+    }
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->done, hipEventDisableTiming);
+    g->pre.assign(n, nullptr);
+    g->join.assign(n, nullptr);
+    for (int i = 0; i < n && e == hipSuccess; i++) {
+        e = hipEventCreateWithFlags(&g->pre[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        free_group(g);
+        return fail(GKM_E_HIP, "group set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return GKM_OK;
+}
+
+int gkm_group_step(gkm_group_t* g, int nsteps) {
+    if (!g) return fail(GKM_E_ARG, "null group");
+    if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
+    const int n = (int)g->hs.size();
+    for (int i = 0; i < n; i++) {
+        Ctx* h = g->hs[i];
+        // every box needs a dt; constants changed since the last upload are
+        // uploaded with the dt values last given
+        std::vector<double> dt(h->nbox);
+        for (int b = 0; b < h->nbox; b++) {
+            dt[b] = h->h_pars[b].dt;
+            if (!(dt[b] > 0))
+                return fail(GKM_E_ARG, "handle %d box %d has no dt: call gkm_set_dt_batch or gkm_step_batch first", i,
+                            b);
+        }
+        if (!h->pars_valid) {
+            int rc = upload_pars(h, dt.data());
+            if (rc) return rc;
+        }
+        // the group's work follows what is queued on the handle
+        HIP_TRY(hipEventRecord(g->pre[i], h->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, g->pre[i], 0));
+    }
+    for (int s = 0; s < nsteps; s++) {
+        std::vector<int> flips(n);
+        for (int i = 0; i < n; i++) flips[i] = g->hs[i]->flip;
+        gkm_group::Cap* cap = nullptr;
+        for (auto& c : g->caps)
+            if (c.flips == flips) cap = &c;
+        if (!cap) {
+            // capture one step of every handle, each on its own stream as a branch
+            gkm_group::Cap c;
+            c.flips = flips;
+            HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+            (void)hipEventRecord(g->fork, g->stream);
+            for (int i = 0; i < n; i++) {
+                Ctx* h = g->hs[i];
+                (void)hipStreamWaitEvent(h->stream, g->fork, 0);
+                gkm::launch_step(h, nullptr);  // swaps the pointers for the work it records
+                (void)hipEventRecord(g->join[i], h->stream);
+                (void)hipStreamWaitEvent(g->stream, g->join[i], 0);
+            }
+            HIP_TRY(hipStreamEndCapture(g->stream, &c.graph));
+            hipError_t e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(c.graph);
+                return fail(GKM_E_HIP, "group graph instantiation failed: %s", hipGetErrorString(e));
+            }
+            g->caps.push_back(c);
+            HIP_TRY(hipGraphLaunch(g->caps.back().exec, g->stream));
+        } else {
+            HIP_TRY(hipGraphLaunch(cap->exec, g->stream));
+            for (Ctx* h : g->hs)
+                if (h->par.flux_scheme == GKM_SCHEME_WENO_BGK) gkm::swap_state(h);
+        }
+    }
+    // the handles' later work follows the group's; ghosts filled for reads
+    HIP_TRY(hipEventRecord(g->done, g->stream));
+    for (Ctx* h : g->hs) {
+        HIP_TRY(hipStreamWaitEvent(h->stream, g->done, 0));
+        gkm::launch_ghost(h, false);
+    }
+    HIP_TRY(hipGetLastError());
+    return GKM_OK;
+}
+
+int gkm_group_sync(gkm_group_t* g) {
+    if (!g) return fail(GKM_E_ARG, "null group");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (Ctx* h : g->hs) HIP_TRY(hipStreamSynchronize(h->stream));
+    return GKM_OK;
+}
+
+int gkm_group_destroy(gkm_group_t* g) {
+    if (!g) return GKM_OK;
+    (void)hipStreamSynchronize(g->stream);
+    free_group(g);
     return GKM_OK;
 }
 

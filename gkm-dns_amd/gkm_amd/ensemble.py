@@ -6,7 +6,11 @@ stage of a step (ghost fill + primitives, three WENO/flux sweeps, update) is a
 single launch over all boxes of the batch, with each box's dt and gas
 constants read from a device table.  Each batch replays its step as a
 captured hipGraph on its own stream, so the size classes run concurrently.
-Launch shapes per size come from the autotuner (tune.py).
+With group=True one captured graph holds a step of every size class instead
+(gkm_group_step), so one graph launch advances the whole ensemble; on MI355X
+that measured ~3 % slower than the per-class streams at the 96-box baseline
+ensemble, so it is not the default.  Launch shapes per size come from the
+autotuner (tune.py).
 """
 from __future__ import annotations
 
@@ -16,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from .ic import dhit
-from .solver import Batch
+from .solver import Batch, Group
 
 
 @dataclass
@@ -41,7 +45,7 @@ def baseline_ensemble(seed: int = 2024) -> list[Realization]:
 class Ensemble:
     def __init__(self, reals: list[Realization], precision: str = "mixed", recon: str = "js",
                  energy_recon: str = "T", visc_law: str = "power", tuning: dict | None = None,
-                 graph: bool = True):
+                 graph: bool = True, group: bool = False):
         self.reals = list(reals)
         self.sizes = sorted({r.n for r in self.reals})
         self.batches: dict[int, Batch] = {}
@@ -57,6 +61,17 @@ class Ensemble:
             self.batches[n] = b
             self.index[n] = idx
         self.cells = sum(r.n ** 3 for r in self.reals)
+        # group: one captured graph holds a step of every size class, so one
+        # graph launch advances the whole ensemble (gkm_group_step)
+        self.group = Group(list(self.batches.values())) if group else None
+
+    def use_group(self, on: bool = True) -> None:
+        """Step through one group graph (True) or each batch on its own stream."""
+        if on and self.group is None:
+            self.group = Group(list(self.batches.values()))
+        elif not on and self.group is not None:
+            self.group.close()
+            self.group = None
 
     def timestep(self, cfl: float) -> dict[int, np.ndarray]:
         return {n: b.timestep(cfl) for n, b in self.batches.items()}
@@ -66,10 +81,15 @@ class Ensemble:
             b.dt[:] = dts[n]
 
     def step(self, nsteps: int = 1) -> None:
+        if self.group is not None:
+            self.group.step(nsteps)
+            return
         for b in self.batches.values():
             b.step(nsteps)
 
     def sync(self) -> None:
+        if self.group is not None:
+            self.group.sync()
         for b in self.batches.values():
             b.sync()
 
@@ -89,5 +109,7 @@ class Ensemble:
         return out
 
     def close(self) -> None:
+        if self.group is not None:
+            self.group.close()
         for b in self.batches.values():
             b.close()

@@ -358,6 +358,13 @@ class Batch:
         _abi.check(self._L.gkm_step_batch(self._h, dt.ctypes.data_as(_abi.DP), int(nsteps)))
         self.t += nsteps * dt
 
+    def set_dt(self, dt=None) -> None:
+        """dt per box for later steps, without stepping (gkm_set_dt_batch)."""
+        if dt is not None:
+            self.dt[:] = np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.nbox,))
+        d = np.ascontiguousarray(self.dt, dtype=np.float64)
+        _abi.check(self._L.gkm_set_dt_batch(self._h, d.ctypes.data_as(_abi.DP)))
+
     def stats(self, box: int) -> dict:
         s = _abi.GkmStats()
         _abi.check(self._L.gkm_stats_box(self._h, int(box), C.byref(s)))
@@ -385,6 +392,45 @@ class Batch:
         if getattr(self, "_h", None) and self._h.value:
             self._L.gkm_destroy(self._h)
             self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Group:
+    """Handles stepped together by one captured graph per step (gkm_group_*):
+    every box of every member (Solver or Batch) advances with one graph
+    launch, each member with the dt it was last given."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self._L = _abi.load()
+        self._g = C.c_void_p()
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        _abi.check(self._L.gkm_group_create(arr, len(self.members), C.byref(self._g)))
+
+    def step(self, nsteps: int = 1) -> None:
+        for m in self.members:  # each member's current dt (uploaded only when it changed)
+            d = np.ascontiguousarray(m.dt if isinstance(m, Batch) else [m.info.dt], dtype=np.float64)
+            _abi.check(self._L.gkm_set_dt_batch(m._h, d.ctypes.data_as(_abi.DP)))
+        _abi.check(self._L.gkm_group_step(self._g, int(nsteps)))
+        for m in self.members:
+            if isinstance(m, Batch):
+                m.t += nsteps * m.dt
+            else:
+                m.info.t += nsteps * m.info.dt
+                m.info.steps += nsteps
+
+    def sync(self) -> None:
+        _abi.check(self._L.gkm_group_sync(self._g))
+
+    def close(self) -> None:
+        if getattr(self, "_g", None) and self._g.value:
+            self._L.gkm_group_destroy(self._g)
+            self._g = C.c_void_p()
 
     def __del__(self):
         try:

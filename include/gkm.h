@@ -155,6 +155,8 @@ int gkm_set_state_box(gkm_t* h, int box, const double* const W[5]);
 int gkm_get_state_box(gkm_t* h, int box, double* const W[5]);
 /* dt[b] per box; nsteps steps of every box. */
 int gkm_step_batch(gkm_t* h, const double* dt, int nsteps);
+/* Set dt[b] per box for later steps without stepping (for gkm_group_step). */
+int gkm_set_dt_batch(gkm_t* h, const double* dt);
 int gkm_compute_dt_box(gkm_t* h, int box, double cfl, double* dt);
 int gkm_stats_box(gkm_t* h, int box, gkm_stats_t* out);
 
@@ -174,6 +176,20 @@ int gkm_set_params_box(gkm_t* h, int box, const gkm_params* p);
  * by default.  Steps timed with gkm_set_timing are launched without the
  * graph. */
 int gkm_set_graph(gkm_t* h, int enabled);
+
+/* A group of handles on the current device stepped together, for example an
+ * ensemble of uneven box sizes with one batch handle per size.  One captured
+ * hipGraph holds one step of every handle, each handle's launches on a branch
+ * of their own, so one graph launch advances every box of every handle.  Each
+ * handle steps with the dt of its last gkm_step_batch or gkm_set_dt_batch.
+ * The handles stay usable on their own between group calls (every group call
+ * is ordered after the work already queued on them, and their later work after
+ * it); destroy the group before its handles.  No reference counterpart. */
+typedef struct gkm_group gkm_group_t;
+int gkm_group_create(gkm_t* const* handles, int n, gkm_group_t** out);
+int gkm_group_step(gkm_group_t* g, int nsteps);
+int gkm_group_sync(gkm_group_t* g);
+int gkm_group_destroy(gkm_group_t* g);
 
 /* Launch shapes: threads per block of the flux kernels (64, 128 or 256) and
  * interfaces per thread of the strided-sweep WENO kernels (4, 8 or 16).

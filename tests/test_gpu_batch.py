@@ -107,3 +107,68 @@ def test_pack_halo_matches_gather(gk, prec):
             assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_group_step_matches_separate_steps(gk, prec):
+    """A group of a 12^3 batch, a 16x12x10 box and a GKS box, stepped by one
+    captured graph per step (gkm_group_step), equals the same handles stepped
+    one by one, bitwise, over both state-buffer parities and with eager work
+    (statistics, a lone step of one member) between the group calls."""
+    import gkm_amd
+    kw = dict(precision=prec, visc_law="power", recon="js")
+    ics = [gk.dhit(12, Mt=0.3 + 0.1 * i, Re=40, seed=80 + i) for i in range(2)]
+    ic1 = gk.dhit(16, Mt=0.4, Re=50, seed=90, ny=12, nz=10)
+    ic2 = gk.dhit(10, Mt=0.3, Re=40, seed=91)
+
+    def make():
+        b = gk.Batch.from_ics(ics, **kw)
+        s1 = gk.Solver.from_ic(ic1, **kw)
+        s2 = gk.Solver.from_ic(ic2, flux_scheme="gks", Pr=1.0, precision=prec, visc_law="power")
+        b.timestep(0.4)
+        s1.timestep(0.4)
+        s2.timestep(0.4)
+        return b, s1, s2
+
+    b, s1, s2 = make()
+    g = gkm_amd.Group([b, s1, s2])
+    g.step(3)
+    b.stats(0)
+    s1.evolve_eulerian(1)  # a lone step flips s1 only: the group recaptures
+    g.step(2)
+    g.sync()
+    got = [b.get_state(0), b.get_state(1), s1.get_state(), s2.get_state()]
+    g.close()
+    for x in (b, s1, s2):
+        x.close()
+
+    b, s1, s2 = make()
+    b.step(3)
+    s1.evolve_eulerian(4)
+    s2.evolve_eulerian(3)
+    b.step(2)
+    s1.evolve_eulerian(2)
+    s2.evolve_eulerian(2)
+    want = [b.get_state(0), b.get_state(1), s1.get_state(), s2.get_state()]
+    for x in (b, s1, s2):
+        x.close()
+    for k, (x, y) in enumerate(zip(got, want)):
+        assert all(np.array_equal(p, q) for p, q in zip(x, y)), k
+
+
+def test_group_needs_dt(gk):
+    import ctypes as C
+
+    import gkm_amd._abi as A
+    s = gk.Solver.from_dhit(8, Mt=0.3, Re=30, seed=1)
+    L = A.load()
+    g = C.c_void_p()
+    arr = (C.c_void_p * 1)(s._h.value)
+    A.check(L.gkm_group_create(arr, 1, C.byref(g)))
+    assert L.gkm_group_step(g, 1) == A.GKM_E_ARG
+    assert b"no dt" in L.gkm_last_error()
+    arr2 = (C.c_void_p * 2)(s._h.value, s._h.value)
+    g2 = C.c_void_p()
+    assert L.gkm_group_create(arr2, 2, C.byref(g2)) == A.GKM_E_ARG
+    L.gkm_group_destroy(g)
+    s.close()
