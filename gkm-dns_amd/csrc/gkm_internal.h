@@ -85,7 +85,7 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& p, double dt);
 void launch_ghost(Ctx* h, bool prim);  // prim = false: ghosts only
 // one step; for the WENO-BGK scheme it leaves the new state in the other
 // buffer and swaps the current-state pointers (swap_state)
-void launch_step(Ctx* h, hipEvent_t* ev);  // ev: 13 events or nullptr
+void launch_step(Ctx* h, hipEvent_t* ev);  // ev: kEvPerStep events or nullptr
 void swap_state(Ctx* h);
 void launch_split(Ctx* h, int box, int q, const double* dsrc);
 void launch_merge(Ctx* h, int box, int q, double* ddst);

@@ -782,16 +782,14 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
         if (ev) (void)hipEventRecord(ev[2], h->stream);
         return;
     }
+    // WENO-BGK: the X and Y sweeps (the Z sweep's flux is fused with the update)
     launch_recon_any<Real, St>(h, dir, st, Tw, wi);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     if (dir == 0)
         hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
-    else if (dir == 1)
-        hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
-                           h->d_pars);
     else
-        hipLaunchKernelGGL((k_flux<Real, St, 2>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+        hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }

@@ -111,6 +111,11 @@ CASES = [
     ((16, 16, 16), dict(), False),
     ((12, 8, 16), dict(recon="js", energy_recon="E", visc_law="power", zbug_compat=False), True),
     ((10, 14, 12), dict(Pr=1.0), True),
+    # the smallest boxes the handle accepts: every ghost is an image of the
+    # interior three cells away, and a z line holds 4 interfaces, so the fused
+    # Z kernel's wavefronts span 15 lines
+    ((3, 3, 3), dict(recon="js", visc_law="power"), True),
+    ((5, 3, 4), dict(), True),
 ]
 
 
