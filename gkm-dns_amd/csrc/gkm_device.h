@@ -311,23 +311,20 @@ template <class T> __device__ __forceinline__ void full_ntab(const Gas<T>& g, T 
     ntab_sums(g, h, t);
 }
 
-// erfcx(a) = exp(a^2) erfc(a), a >= 0, FP32: the polynomial of gkm_erfcx.h on
-// [0, kErfcxB]; beyond it (erfc < 2.2e-17) the asymptotic series
-// (1 - 1/(2a^2) + 3/(4a^4) - 15/(8a^6)) / (a sqrt(pi)).  Branch-free.
-template <class T> __device__ __forceinline__ T erfcx_tail(T a) {
-    const T ia = M<T>::rcp(a), i2 = ia * ia;
-    const T s = fma(fma(fma(T(-1.875), i2, T(0.75)), i2, T(-0.5)), i2, T(1));
-    return s * ia * T(0.56418958354775628694807945156077259);  // 1/sqrt(pi)
-}
+// erfcx(a) = exp(a^2) erfc(a), a >= 0, FP32: the rational P(a)/Q(a) of
+// gkm_erfcx.h (deg 4 / deg 5, positive coefficients, so Horner's rule has no
+// cancellation), a capped at kErfcxCap where exp(-a^2) is already 0 in FP32.
 __device__ __forceinline__ float erfcx_(float a) {
-    const float u = fminf(a, float(kErfcxB)) * float(2.0 / kErfcxB) - 1.0f;
-    float p = kErfcxF[kErfcxDegF];
+    const float x = fminf(a, float(kErfcxCap));
+    float p = kErfcxP[kErfcxDegP], q = kErfcxQ[kErfcxDegQ];
 #pragma unroll
-    for (int i = kErfcxDegF - 1; i >= 0; i--) p = fmaf(p, u, kErfcxF[i]);
-    return a > float(kErfcxB) ? erfcx_tail(a) : p;
+    for (int i = kErfcxDegP - 1; i >= 0; i--) p = fmaf(p, x, kErfcxP[i]);
+#pragma unroll
+    for (int i = kErfcxDegQ - 1; i >= 0; i--) q = fmaf(q, x, kErfcxQ[i]);
+    return p * M<float>::rcp(q);
 }
 
-// erfc(y) given e = exp(-y^2).  FP32: erfcx polynomial times e, branch-free.
+// erfc(y) given e = exp(-y^2).  FP32: erfcx rational times e, branch-free.
 // FP64: the device library's erfc (a 35-term FP64 polynomial needs its
 // coefficients in registers, which the FP64 flux kernel, already at 256 VGPRs,
 // cannot spare without spilling).
