@@ -282,15 +282,17 @@ def bench_ensemble(args):
 def bench_slab(args):
     """One rank's 1024 x 1024 x 128 slab of a 1024^3 box (Mt = 0.6, Re = 150), the
     y faces exchanged by on-device pack / unpack (timed apart from the compute),
-    mixed precision; one FP64 step from the same state checks the increment."""
+    mixed precision; one FP64 step from the same state checks the increment.
+    The initial state is the init_BS.m recipe on the slab's periodic grid,
+    generated by the library's host generator (gkm_dhit_velocity)."""
     import numpy as np
 
     import gkm_amd
-    from gkm_amd.ic import conservative_padded, synthetic_solenoidal
+    from gkm_amd.ic import conservative_padded, dhit_native
 
     nx, ny, nz = 1024, 128, 1024
     t0 = time.perf_counter()
-    ic = synthetic_solenoidal(nx, ny, nz, Mt=0.6, Re=150.0)
+    ic = dhit_native(nx, k0=4, Mt=0.6, Re=150.0, seed=1234, ny=ny, nz=nz)
     p0, mu0, rho0 = ic.p0, ic.mu0, ic.rho0
     W0 = conservative_padded(ic)
     del ic

@@ -148,47 +148,3 @@ def conservative_padded(ic: DhitIC, ghost: int = 3):
     W[3][s] = ic.w * d0
     W[4][s] = d0 * (pr0 / (d0 * (ic.gamma - 1.0)) + 0.5 * vsqr)
     return [np.ascontiguousarray(x.reshape(-1)) for x in W]
-
-
-def synthetic_solenoidal(nx: int, ny: int, nz: int, Mt: float = 0.6, k0: float = 4.0, nmodes: int = 4,
-                         seed: int = 1234, Re: float = 150.0, rho0: float = 1.0, gamma: float = 1.4) -> DhitIC:
-    """Cheap divergence-free field for throughput runs on large boxes (where
-    the FFT recipe above would dominate the set-up): u = sum_m A_m cos(k_m . x
-    + phi_m) with integer wavevectors |k_m| ~ k0 on the periodic box and
-    A_m = k_m x e_m (so k_m . A_m = 0), normalised like dhit() (c0 =
-    sqrt(3) u_rms / Mt, p0 = rho0 c0^2 / gamma, mu0 = u_rms (2/k0) rho0 / Re).
-    Evaluated one y plane at a time to bound host memory."""
-    rng = np.random.default_rng(seed)
-    ks, amps, phs = [], [], []
-    while len(ks) < nmodes:
-        k = rng.integers(-int(k0) - 1, int(k0) + 2, 3)
-        if np.linalg.norm(k) < 1:
-            continue
-        a = np.cross(k, rng.normal(size=3))
-        ks.append(k.astype(np.float64))
-        amps.append(a / np.linalg.norm(a))
-        phs.append(rng.uniform(0, 2 * np.pi))
-    x = 2 * np.pi * np.arange(nx) / nx
-    z = 2 * np.pi * np.arange(nz) / nz
-    y = 2 * np.pi * np.arange(ny) / ny
-    u = np.empty((ny, nx, nz))
-    v = np.empty((ny, nx, nz))
-    w = np.empty((ny, nx, nz))
-    for j in range(ny):
-        su = np.zeros((nx, nz))
-        sv = np.zeros((nx, nz))
-        sw = np.zeros((nx, nz))
-        for k, a, ph in zip(ks, amps, phs):
-            # k = (kx, ky, kz); cos(kx x + ky y_j + kz z + ph) = Re(e^{i kx x} e^{i kz z} e^{i(ky y_j + ph)})
-            c = np.cos(k[0] * x[:, None] + k[2] * z[None, :] + k[1] * y[j] + ph)
-            su += a[0] * c
-            sv += a[1] * c
-            sw += a[2] * c
-        u[j], v[j], w[j] = su, sv, sw
-    q2 = float(np.mean(u * u) + np.mean(v * v) + np.mean(w * w))
-    urms = np.sqrt(q2 / 3.0)
-    c0 = np.sqrt(3.0) * urms / Mt
-    p0 = rho0 * c0 * c0 / gamma
-    lam = 2.0 / k0
-    return DhitIC(u=u, v=v, w=w, rho0=rho0, p0=p0, mu0=urms * lam * rho0 / Re, t0=lam / urms, urms=urms, c0=c0,
-                  Mt=Mt, Re=Re, k0=k0, gamma=gamma)
