@@ -360,7 +360,7 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
                 h->graph_ready[f] = true;
             }
             HIP_TRY(hipGraphLaunch(h->graph_exec[f], h->stream));
-            if (!capture && h->par.flux_scheme == GKM_SCHEME_WENO_BGK) gkm::swap_state(h);
+            if (!capture) gkm::swap_state(h);
         } else {
             gkm::launch_step(h, nullptr);
         }
@@ -536,8 +536,7 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
             HIP_TRY(hipGraphLaunch(g->caps.back().exec, g->stream));
         } else {
             HIP_TRY(hipGraphLaunch(cap->exec, g->stream));
-            for (Ctx* h : g->hs)
-                if (h->par.flux_scheme == GKM_SCHEME_WENO_BGK) gkm::swap_state(h);
+            for (Ctx* h : g->hs) gkm::swap_state(h);
         }
     }
     // the handles' later work follows the group's; ghosts filled for reads

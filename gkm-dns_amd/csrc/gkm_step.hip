@@ -376,13 +376,10 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
 // in FP64 (T, p and tau are recovered in FP64 in both forms); the tangential
 // neighbours only enter slope differences and are read as cell values.
 // ---------------------------------------------------------------------------
+// the GKM tree's smooth-flow flux at interface c (rotated slot order in Fs)
 template <class Real, class St, int DIR>
-__global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st,
-                                             Planes5<Real> F, const StepPar* __restrict__ pars) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const StepPar& par = pars[blockIdx.y];
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+__device__ __forceinline__ void gks_at(unsigned c, unsigned sn, unsigned st1, unsigned st2, const St& st,
+                                       const StepPar& par, Real Fs[5]) {
     Gas<Real> gas;
     gas.gam = upar<Real>(par, kUGam);
     gas.invPr = upar<Real>(par, kUInvPr);
@@ -410,36 +407,26 @@ __global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigne
     const double r = w0[0], u = w0[1] / r, v = w0[2] / r, w = w0[3] / r;
     const double P = (r * (par.gam - 1)) * (w0[4] / r - 0.5 * (u * u + v * v + w * w));
     const Real tau = collision_time<Real>(par, P / (par.R * r), P);
-    Real Fs[5];
     gks_flux(gas, W0, Dx, Dy, Dz, tau, upar<Real>(par, kUDt), Fs);
+}
+
+template <class Real, class St, int DIR>
+__global__ void __launch_bounds__(256) k_gks(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st,
+                                             Planes5<Real> F, const StepPar* __restrict__ pars) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    Real Fs[5];
+    gks_at<Real, St, DIR>(c, sn, st1, st2, st, par, Fs);
 #pragma unroll
     for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
 
 // ---------------------------------------------------------------------------
-// Conservative update on the interior (update.cpp:455-469): the flux
+// Conservative update of one interior cell (update.cpp:455-469): the flux
 // differences are summed in the reference's order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_update_d(Geo g, Box3 b, StateD S, Planes15<double> F, double dx) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
-    // every load before the first store: the state planes may alias the flux
-    // planes as far as the compiler knows, so interleaving would serialise
-    // the five fields' memory round trips
-    double d[5], w[5];
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        const double* Fx = F.p[q];
-        const double* Fy = F.p[5 + q];
-        const double* Fz = F.p[10 + q];
-        d[q] = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
-        w[q] = ld(S.w[q], c);
-    }
-#pragma unroll
-    for (int q = 0; q < 5; q++) put(S.w[q], c, w[q] - (1 / dx) * d[q]);
-}
-
 // Mixed form: the FP32 flux divergence is added to the hi/lo pair with an
 // error-free TwoSum followed by a Fast2Sum renormalisation.
 __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
@@ -453,7 +440,9 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
 }
 
 // new value of interior cell c into the other buffer: x and y flux planes of
-// the earlier sweeps, z fluxes of the cell's right (Fz) and left (Fm) faces
+// the earlier sweeps, z fluxes of the cell's right (Fz) and left (Fm) faces.
+// Every load comes before the first store (the planes may alias as far as the
+// compiler knows, and interleaving would serialise the fields' round trips).
 __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateD& S, const StateD& N,
                                             const Planes15<double>& F, const double Fz[5], const double Fm[5],
                                             const StepPar& par) {
@@ -486,28 +475,6 @@ __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const Stat
     }
 }
 
-__global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM S, Planes15<float> F, float idx) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
-    float R[5], hi[5], lo[5];  // all loads first (see k_update_d)
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        const float* Fx = F.p[q];
-        const float* Fy = F.p[5 + q];
-        const float* Fz = F.p[10 + q];
-        R[q] = ld(Fx, c) - ld(Fx, c - g.s1) + ld(Fy, c) - ld(Fy, c - g.s0) + ld(Fz, c) - ld(Fz, c - 1);
-        hi[q] = ld(S.hi[q], c);
-        lo[q] = ld(S.lo[q], c);
-    }
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        two_sum_add(hi[q], lo[q], -idx * R[q]);
-        put(S.hi[q], c, hi[q]);
-        put(S.lo[q], c, lo[q]);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Z sweep fused with the conservative update.  Interfaces are numbered along
 // the z lines of the interior (j, i) columns, interface k+1/2 for k in
@@ -516,9 +483,10 @@ __global__ void __launch_bounds__(256) k_update_m(Geo g, Box3 b, StateM S, Plane
 // wavefronts share one: lane l computes the flux at its interface, takes its
 // left neighbour's from lane l-1 (__shfl_up), and for l >= 1 and k >= 3
 // updates cell k from the x, y fluxes of the earlier sweeps and the two z
-// fluxes, in the summation order of k_update_d / k_update_m.  The state is
-// read from the current buffer and the new interior written to the other
-// one, so no lane's read of a neighbour cell can see an updated value.
+// fluxes (update_cell).  The state is read from the current buffer and the
+// new interior written to the other one, so no lane's read of a neighbour
+// cell can see an updated value.  The flux is the WENO-BGK one (flux_at, from
+// the interface planes) or the GKS one (gks_at, from the cells).
 // ---------------------------------------------------------------------------
 // Lines are taken in strips of W columns of i, (j, i) within a strip, so a
 // line's y neighbours are W lines away in launch order instead of nx; with the
@@ -543,7 +511,7 @@ __device__ __forceinline__ void zline_column(const ZLines& zl, unsigned l, unsig
     i = s * zl.fW.d + (r - j * fw.d);
 }
 
-template <class Real, class St>
+template <class Real, class St, int SCHEME>
 __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
                   Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars) {
@@ -569,7 +537,10 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     Real Fz[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
     if (valid) {
         Real Fs[5];
-        flux_at<Real, St, 2>(c, 1u, st1, st2, st, tau, wi, par, Fs);
+        if constexpr (SCHEME == GKM_SCHEME_GKS)
+            gks_at<Real, St, 2>(c, 1u, st1, st2, st, par, Fs);
+        else
+            flux_at<Real, St, 2>(c, 1u, st1, st2, st, tau, wi, par, Fs);
 #pragma unroll
         for (int s = 0; s < 5; s++) Fz[slot_field<2>(s)] = Fs[s];
     }
@@ -696,7 +667,6 @@ static Box3 sweep_box(const Geo& g, int an, bool extended) {
     return make_box(lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1);
 }
 
-static Box3 interior_box(const Geo& g) { return make_box(3, 3, 3, g.ny, g.nx, g.nz); }
 static Box3 padded_box(const Geo& g) { return make_box(0, 0, 0, g.P0, g.P1, g.P2); }
 
 // axis of the sweep direction: X -> 1, Y -> 0, Z -> 2
@@ -794,8 +764,8 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
-// Z sweep of the WENO-BGK scheme: reconstruction, then the flux fused with
-// the update of every interior cell into the other state buffer
+// Z sweep: reconstruction (WENO-BGK), then the flux fused with the update of
+// every interior cell into the other state buffer
 template <class Real, class St>
 static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Real* tau, Planes10<Real> wi,
                                   Planes15<Real> F, hipEvent_t* ev) {
@@ -817,10 +787,16 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     const int bs = h->flux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_recon_any<Real, St>(h, 2, st, Tw, wi);
-    if (ev) (void)hipEventRecord(ev[1], h->stream);
-    hipLaunchKernelGGL((k_flux_update<Real, St>), grid, dim3(bs), 0, h->stream, g, zl, st1, st2, st, nx, tau, wi, F,
-                       h->d_pars);
+    if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction
+        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_GKS>), grid, dim3(bs), 0, h->stream, g, zl, st1, st2,
+                           st, nx, tau, wi, F, h->d_pars);
+    } else {
+        launch_recon_any<Real, St>(h, 2, st, Tw, wi);
+        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_WENO_BGK>), grid, dim3(bs), 0, h->stream, g, zl, st1,
+                           st2, st, nx, tau, wi, F, h->d_pars);
+    }
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
@@ -915,67 +891,51 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
 }
 
 void launch_step(Ctx* h, hipEvent_t* ev) {
-    const Geo& g = h->geo;
-    // the GKS scheme reads no per-cell primitive planes; the WENO-BGK scheme
-    // fuses the update into its Z sweep and writes the other state buffer
+    // the GKS scheme reads no per-cell primitive planes; both schemes fuse the
+    // update into their Z sweep, which writes the other state buffer
     const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
-    const bool fused = h->par.flux_scheme == GKM_SCHEME_WENO_BGK;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     launch_ghost(h, prim);
     if (ev) (void)hipEventRecord(ev[12], h->stream);
     if (prim && h->halo_mode == GKM_HALO_PACK) launch_prim(h);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
-    Box3 bi = interior_box(g);
-    const dim3 ugrid(blocks_for(bi.count(), 256), h->nbox);
     if (h->par.precision == GKM_PREC_FP64) {
-        StateD st;
-        for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
+        StateD st, nx;
+        for (int q = 0; q < 5; q++) {
+            st.w[q] = h->Wd[q];
+            nx.w[q] = h->Wd_nx[q];
+        }
         Planes10<double> wi;
         for (int q = 0; q < 10; q++) wi.p[q] = h->WId[q];
         Planes15<double> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Fd[q];
-        const int nsep = fused ? 2 : 3;  // sweeps with a separate flux kernel
-        for (int dir = 0; dir < nsep; dir++) {
+        for (int dir = 0; dir < 2; dir++) {  // X and Y; Z is fused with the update
             Planes5<double> Fd;
             for (int q = 0; q < 5; q++) Fd.p[q] = h->Fd[5 * dir + q];
             launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, Fd, ev ? ev + 1 + 3 * dir : nullptr);
         }
-        if (fused) {
-            StateD nx;
-            for (int q = 0; q < 5; q++) nx.w[q] = h->Wd_nx[q];
-            launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr);
-        } else {
-            hipLaunchKernelGGL(k_update_d, ugrid, dim3(256), 0, h->stream, g, bi, st, F, h->grid.dx);
-        }
+        launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr);
     } else {
-        StateM st;
+        StateM st, nx;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q];
             st.lo[q] = h->Wlo[q];
+            nx.hi[q] = h->Whi_nx[q];
+            nx.lo[q] = h->Wlo_nx[q];
         }
         Planes10<float> wi;
         for (int q = 0; q < 10; q++) wi.p[q] = h->WIf[q];
         Planes15<float> F;
         for (int q = 0; q < 15; q++) F.p[q] = h->Ff[q];
-        const int nsep = fused ? 2 : 3;
-        for (int dir = 0; dir < nsep; dir++) {
+        for (int dir = 0; dir < 2; dir++) {
             Planes5<float> Ff;
             for (int q = 0; q < 5; q++) Ff.p[q] = h->Ff[5 * dir + q];
             launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, Ff, ev ? ev + 1 + 3 * dir : nullptr);
         }
-        if (fused) {
-            StateM nx;
-            for (int q = 0; q < 5; q++) {
-                nx.hi[q] = h->Whi_nx[q];
-                nx.lo[q] = h->Wlo_nx[q];
-            }
-            launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr);
-        } else {
-            hipLaunchKernelGGL(k_update_m, ugrid, dim3(256), 0, h->stream, g, bi, st, F, (float)(1.0 / h->grid.dx));
-        }
+        launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr);
     }
     if (ev) (void)hipEventRecord(ev[11], h->stream);
-    if (fused) swap_state(h);
+    swap_state(h);
 }
 
 void launch_split(Ctx* h, int box, int q, const double* dsrc) {

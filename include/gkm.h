@@ -250,9 +250,9 @@ int gkm_sync(gkm_t* h);
 int gkm_set_timing(gkm_t* h, int enabled);
 int gkm_last_timing(gkm_t* h, double ms[5], int* nsteps);
 /* Flux-kernel time of each sweep direction (X, Y, Z) over the timed steps of
- * the last gkm_step call; in the WENO-BGK scheme the Z entry is the flux
- * kernel that also applies the conservative update (so ms[3] of
- * gkm_last_timing is ~0 there).  -1 where nothing was recorded. */
+ * the last gkm_step call; the Z entry is the flux kernel that also applies
+ * the conservative update (so ms[3] of gkm_last_timing is ~0).  -1 where
+ * nothing was recorded. */
 int gkm_last_sweep_timing(gkm_t* h, double ms[3]);
 
 /* Ghost-fill mode (GKM_HALO_*).  The y-face pack/unpack path is the one a
