@@ -24,6 +24,17 @@ struct Geo {
     unsigned ncell;   // padded cells of one box; box b's planes start at b * ncell
 };
 
+// Blocks are dealt to the 8 XCDs round-robin by blockIdx.x.  The logical block
+// number below gives XCD x the x-th contiguous eighth of the blocks, so that
+// neighbouring rows of interfaces (or grid lines), which read each other's
+// values or share cache lines, run on one XCD and share its L2.  A bijection on [0, gridDim.x).  (For a
+// batch, blockIdx.y > 0 rows start on other XCDs when gridDim.x is not a
+// multiple of 8; the mapping stays a bijection, only the locality is lost.)
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned nb = gridDim.x, xcd = blockIdx.x & 7u, q = nb >> 3, rem = nb & 7u;
+    return xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
+}
+
 // Division by a run-time invariant divisor d >= 1 for dividends t < 2^31:
 // q = (umulhi(t, m) + t) >> s with s = ceil(log2 d),
 // m = floor(2^32 (2^s - d) / d) + 1 (round-up reciprocal, 33-bit magic).

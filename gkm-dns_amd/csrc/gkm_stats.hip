@@ -9,6 +9,9 @@
 // partials in a fixed order, so the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "gkm_internal.h"
 
 namespace gkm {
@@ -28,9 +31,10 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Reduce NQ per-thread values over the block; thread 0 writes the partials.
 // Slots listed in MAXMASK are max-reduced, the others summed.
-template <int NQ, unsigned MAXMASK>
+// Blocks of NW wavefronts.
+template <int NQ, unsigned MAXMASK, int NW = kRedThreads / 64>
 __device__ __forceinline__ void block_reduce_store(double (&acc)[NQ], double* part) {
-    __shared__ double lds[NQ][kRedThreads / 64];
+    __shared__ double lds[NQ][NW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
@@ -41,7 +45,7 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NQ], double* pa
     if (threadIdx.x < NQ) {
         const int q = threadIdx.x;
         double v = lds[q][0];
-        for (int w = 1; w < kRedThreads / 64; w++) v = ((MAXMASK >> q) & 1u) ? fmax(v, lds[q][w]) : v + lds[q][w];
+        for (int w = 1; w < NW; w++) v = ((MAXMASK >> q) & 1u) ? fmax(v, lds[q][w]) : v + lds[q][w];
         part[(long)blockIdx.x * NQ + q] = v;
     }
 }
@@ -241,162 +245,186 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
 // ---------------------------------------------------------------------------
 // PrintStats with spline gradients (weno/src/update.cpp:638-784).
 //
-// k_spline<A>: one thread per grid line along storage axis A (0 = y, 1 = x,
-// 2 = z) through interior nodes.  It walks the whole padded line once forward
-// and once back, solving for the slopes of the natural cubic spline through
-// u, v, w = (rho u, rho v, rho w)/rho at its m nodes:
+// Along every grid line of each axis (through interior nodes of the other
+// two), the slopes D of the natural cubic spline through u, v, w =
+// (rho u, rho v, rho w)/rho at the line's m padded nodes:
 //   2 D_0 + D_1 = (3/h)(f_1 - f_0)
 //   D_{i-1} + 4 D_i + D_{i+1} = (3/h)(f_{i+1} - f_{i-1})
 //   D_{m-2} + 2 D_{m-1} = (3/h)(f_{m-1} - f_{m-2})
-// The matrix is the same for every line of the axis, so the Thomas pivots
+// The matrix is the same for every line of an axis, so the Thomas pivots
 // inv_i = 1/(b_i - inv_{i-1}) come precomputed: forward y_i = (r_i - y_{i-1})
 // inv_i, backward D_i = y_i - inv_i D_{i+1}.  The forward values of nodes
-// 3..m-1 go to the gradient plane and are overwritten by the slopes on the
-// way back; only interior nodes' slopes are used.
-// grad planes: (3 A + comp) * ncell.
+// 3..m-1 go to the axis's gradient plane ((3 A + comp) * ncell; A = 0 y,
+// 1 x, 2 z) and are overwritten by the slopes on the way back; only interior
+// nodes' slopes are used.
+//
+// Each axis is one pass that writes its slopes for the interior nodes to its
+// three planes; k_pstats then forms the sums from the nine planes and the
+// state.  Lines of up to kSplMaxM nodes (boxes up to 335 cells a side) are
+// solved whole in LDS (k_spl_lines, below); longer ones go in chunks through
+// the gradient planes (k_spl_chunked): blocks of 64 lines with three
+// wavefronts, wavefront q solving for velocity component q, so that a pass
+// over 256 x 256 lines puts three wavefronts on every SIMD.
 // ---------------------------------------------------------------------------
-template <class St, int A>
-__global__ void __launch_bounds__(256) k_spline(Geo g, St st, const double* __restrict__ inv, double r3h,
-                                                double* __restrict__ grad) {
-    const int m = A == 0 ? g.P0 : (A == 1 ? g.P1 : g.P2);
-    // lines: the two other axes over their interior ranges, the faster one inner
-    const int n_in = A == 2 ? g.nx : g.nz;
-    const int n_out = A == 0 ? g.nx : g.ny;
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)n_in * (unsigned)n_out) return;
-    const unsigned a_in = t % (unsigned)n_in + 3, a_out = t / (unsigned)n_in + 3;
-    unsigned base, stride;
-    if (A == 0) {  // y lines over (x, z)
-        base = a_out * g.s1 + a_in;
-        stride = g.s0;
-    } else if (A == 1) {  // x lines over (y, z)
-        base = a_out * g.s0 + a_in;
-        stride = g.s1;
-    } else {  // z lines over (y, x)
-        base = a_out * g.s0 + a_in * g.s1;
-        stride = 1;
-    }
-    double* G0 = grad + (size_t)(3 * A) * g.ncell;
-    double* G1 = G0 + g.ncell;
-    double* G2 = G1 + g.ncell;
-    auto velo = [&](unsigned c, double f[3]) {
-        const double r = st.get(0, c);
-        f[0] = st.get(1, c) / r;
-        f[1] = st.get(2, c) / r;
-        f[2] = st.get(3, c) / r;
-    };
-    double fm[3], f0[3], fp[3], y[3] = {0.0, 0.0, 0.0};
-    velo(base, f0);
-    velo(base + stride, fp);
-    fm[0] = f0[0]; fm[1] = f0[1]; fm[2] = f0[2];
-    for (int i = 0; i < m; i++) {
-        const unsigned c = base + (unsigned)i * stride;
-        const double w = inv[i];
+constexpr int kSpLines = 64, kSpThreads = 192, kSpChunk = 16;
+
+// PrintStats sums over the interior (update.cpp:658-707):
+//  0 rho|u|^2  1 mu du_i/dx_j du_i/dx_j  2 S_ij S_ij  3..5 (du/dx)^2,3,4
+//  6..8 (dv/dy)^n  9..11 (dw/dz)^n  12 p  13 T  14 1/rho
+// dX[m] = d(u_m)/dx, dY[m] = d(u_m)/dy, dZ[m] = d(u_m)/dz
+template <class St>
+__device__ __forceinline__ void pstat_add(const St& st, unsigned c, const StatPar& p, const double dX[3],
+                                          const double dY[3], const double dZ[3], double (&acc)[kPStat]) {
+    const double rho = st.get(0, c);
+    double u[3];
+    vel(st, c, u);
+    const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
+    const double T = P / (p.R * rho);
+    const double dudx = dX[0], dudy = dY[0], dudz = dZ[0];
+    const double dvdx = dX[1], dvdy = dY[1], dvdz = dZ[1];
+    const double dwdx = dX[2], dwdy = dY[2], dwdz = dZ[2];
+    const double mu = visc(p, T);
+    acc[0] += rho * q2;
+    acc[1] += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                    dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
+    acc[2] += dudx * dudx + 0.5 * (dudy + dvdx) * (dudy + dvdx) + 0.5 * (dudz + dwdx) * (dudz + dwdx) +
+              0.5 * (dwdy + dvdz) * (dwdy + dvdz) + dvdy * dvdy + dwdz * dwdz;
+    double x2 = dudx * dudx; acc[3] += x2; acc[4] += x2 * dudx; acc[5] += x2 * x2;
+    x2 = dvdy * dvdy; acc[6] += x2; acc[7] += x2 * dvdy; acc[8] += x2 * x2;
+    x2 = dwdz * dwdz; acc[9] += x2; acc[10] += x2 * dwdz; acc[11] += x2 * x2;
+    acc[12] += P; acc[13] += T; acc[14] += 1.0 / rho;
+}
+
+// component q of the velocity at c
+template <class St> __device__ __forceinline__ double velq(const St& st, unsigned c, int q) {
+    return st.get(1 + q, c) / st.get(0, c);
+}
+
+// The walks are latency-bound (one lane per line, each node a dependent
+// step), so they go in chunks of kSpChunk nodes: the chunk's loads are issued
+// together, then the recursion runs through them.
+
+// Forward walk of one line (base, stride, m nodes) for component q: y_i of
+// nodes 3..m-1 into plane G; returns y_{m-1} (= D_{m-1}).
+template <class St>
+__device__ __forceinline__ double spline_forward(const St& st, unsigned base, unsigned stride, int m, int q,
+                                                 const double* __restrict__ inv, double r3h, double* G) {
+    double f0 = velq(st, base, q), fm = f0, y = 0.0;
+    for (int i0 = 0; i0 < m; i0 += kSpChunk) {
+        double fn[kSpChunk];  // f at nodes i0+1 .. i0+kSpChunk
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            // one-sided differences at the two ends, central inside
-            const double hi = i == m - 1 ? f0[q] : fp[q];
-            const double lo = i == 0 ? f0[q] : fm[q];
-            y[q] = (r3h * (hi - lo) - y[q]) * w;
+        for (int k = 0; k < kSpChunk; k++) {
+            const int n = i0 + 1 + k;
+            fn[k] = n < m ? velq(st, base + (unsigned)n * stride, q) : 0.0;
         }
-        if (i >= 3) {
-            put(G0, c, y[0]);
-            put(G1, c, y[1]);
-            put(G2, c, y[2]);
-        }
-        if (i + 1 < m) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                fm[q] = f0[q];
-                f0[q] = fp[q];
+        for (int k = 0; k < kSpChunk; k++) {
+            const int i = i0 + k;
+            if (i < m) {
+                // one-sided differences at the two ends, central inside
+                const double hi = i == m - 1 ? f0 : fn[k];
+                const double lo = i == 0 ? f0 : fm;
+                y = (r3h * (hi - lo) - y) * inv[i];
+                if (i >= 3) put(G, base + (unsigned)i * stride, y);
+                fm = f0;
+                f0 = fn[k];
             }
-            if (i + 2 < m) velo(c + 2 * stride, fp);
         }
     }
-    // y now holds y_{m-1} = D_{m-1}; back-substitute down to node 3
-    double d[3] = {y[0], y[1], y[2]};
-    for (int i = m - 2; i >= 3; i--) {
-        const unsigned c = base + (unsigned)i * stride;
-        const double w = inv[i];
-        d[0] = ld(G0, c) - w * d[0];
-        d[1] = ld(G1, c) - w * d[1];
-        d[2] = ld(G2, c) - w * d[2];
-        put(G0, c, d[0]);
-        put(G1, c, d[1]);
-        put(G2, c, d[2]);
+    return y;
+}
+
+// Backward walk from D_{m-1} = d over nodes m-2 .. 3 of the same line: the
+// slopes overwrite the forward values in G.
+__device__ __forceinline__ void spline_backward(unsigned base, unsigned stride, int m,
+                                                const double* __restrict__ inv, double d, double* G) {
+    for (int i1 = m - 2; i1 >= 3; i1 -= kSpChunk) {
+        double yv[kSpChunk];  // forward values of nodes i1, i1-1, ...
+#pragma unroll
+        for (int k = 0; k < kSpChunk; k++) {
+            const int n = i1 - k;
+            yv[k] = n >= 3 ? ld(G, base + (unsigned)n * stride) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kSpChunk; k++) {
+            const int n = i1 - k;
+            if (n >= 3) {
+                d = yv[k] - inv[n] * d;
+                put(G, base + (unsigned)n * stride, d);
+            }
+        }
     }
 }
 
-// z lines run along the unit stride, so one thread per line would touch a
-// different cache line on every lane.  k_spline_z gives a 64-lane block 64
-// lines and walks them in chunks of 16 nodes staged through LDS: the block
-// loads and stores 16-node row segments (128 B of doubles per line), and
-// each lane runs the same Thomas recursion as k_spline on its own line out
-// of LDS.
-constexpr int kSzLines = 64, kSzChunk = 16;
+// line (b, lane) of a strided axis (A = 0: y lines over (x, z); A = 1: x
+// lines over (y, z); z inner) for component q
+template <class St, int A>
+__device__ __forceinline__ void spline_strided_block(const Geo& g, const St& st, const double* __restrict__ inv,
+                                                     double r3h, double* __restrict__ grad, int b) {
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const unsigned n_out = A == 0 ? g.nx : g.ny;
+    const unsigned t = (unsigned)b * kSpLines + lane;
+    if (t >= n_out * g.nz) return;  // no barriers in this pass
+    const unsigned base = (t / g.nz + 3) * (A == 0 ? g.s1 : g.s0) + (t % g.nz + 3);
+    const unsigned stride = A == 0 ? g.s0 : g.s1;
+    const int m = A == 0 ? g.P0 : g.P1;
+    double* G = grad + (size_t)(3 * A + q) * g.ncell;
+    const double d = spline_forward(st, base, stride, m, q, inv, r3h, G);
+    spline_backward(base, stride, m, inv, d, G);
+}
 
+// z lines run along the unit stride, so a lane per line would touch a
+// different cache line on every lane.  The block walks its 64 lines in chunks
+// of 16 nodes staged through LDS: all 192 threads load and store 16-node row
+// segments (coalesced), and lane l of wavefront q runs the recursion of
+// component q of line l out of the tile.
 template <class St>
-__global__ void __launch_bounds__(kSzLines) k_spline_z(Geo g, St st, const double* __restrict__ inv, double r3h,
-                                                       double* __restrict__ grad) {
-    __shared__ double tile[3][kSzLines][kSzChunk + 1];
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void spline_z_block(const Geo& g, const St& st, const double* __restrict__ inv,
+                                               double r3h, double* __restrict__ grad, int b,
+                                               double (&tile)[3][kSpLines][kSpChunk + 1]) {
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int m = g.P2;
     const int nline = g.ny * g.nx;
-    const int line0 = blockIdx.x * kSzLines;
+    const int line0 = b * kSpLines;
     double* G[3] = {grad + (size_t)6 * g.ncell, grad + (size_t)7 * g.ncell, grad + (size_t)8 * g.ncell};
     auto line_base = [&](int id) -> unsigned {
         const unsigned j = (unsigned)(id / g.nx) + 3, i = (unsigned)(id % g.nx) + 3;
         return j * g.s0 + i * g.s1;
     };
     const bool mine = line0 + lane < nline;
-    const unsigned base = mine ? line_base(line0 + lane) : 0u;
-    // cooperative pass over the block's lines x chunk nodes [n0, n0 + kSzChunk):
-    // element e = r * 64 + lane covers line e / kSzChunk, node e % kSzChunk
+    // cooperative pass over the block's lines x chunk nodes [n0, n0 + kSpChunk):
+    // element e covers line e / kSpChunk, node e % kSpChunk
     auto coop = [&](int n0, auto&& body) {
-#pragma unroll 4
-        for (int r = 0; r < kSzLines * kSzChunk / kSzLines; r++) {
-            const int e = r * kSzLines + lane;
-            const int l = e / kSzChunk, t = e % kSzChunk;
+        for (int e = threadIdx.x; e < kSpLines * kSpChunk; e += kSpThreads) {
+            const int l = e / kSpChunk, t = e % kSpChunk;
             const int n = n0 + t;
             if (line0 + l < nline && n < m) body(l, t, line_base(line0 + l) + (unsigned)n);
         }
     };
-    auto velo = [&](unsigned c, double f[3]) {
-        const double r = st.get(0, c);
-        f[0] = st.get(1, c) / r;
-        f[1] = st.get(2, c) / r;
-        f[2] = st.get(3, c) / r;
-    };
-    double f0[3] = {0.0, 0.0, 0.0}, fm[3], y[3] = {0.0, 0.0, 0.0};
-    if (mine) velo(base, f0);
-    fm[0] = f0[0]; fm[1] = f0[1]; fm[2] = f0[2];
-    const int nch = (m + kSzChunk - 1) / kSzChunk;
+    double f0 = mine ? velq(st, line_base(line0 + lane), q) : 0.0, fm = f0, y = 0.0;
+    const int nch = (m + kSpChunk - 1) / kSpChunk;
     // forward: the tile holds f at nodes n0+1 .. n0+16; lane l walks nodes
     // n0 .. n0+15 of its line and leaves y_n where it read f_{n+1}
     for (int ch = 0; ch < nch; ch++) {
-        const int n0 = ch * kSzChunk;
+        const int n0 = ch * kSpChunk;
         coop(n0 + 1, [&](int l, int t, unsigned c) {
-            double f[3];
-            velo(c, f);
-            tile[0][l][t] = f[0];
-            tile[1][l][t] = f[1];
-            tile[2][l][t] = f[2];
+            const double r = st.get(0, c);
+            tile[0][l][t] = st.get(1, c) / r;
+            tile[1][l][t] = st.get(2, c) / r;
+            tile[2][l][t] = st.get(3, c) / r;
         });
         __syncthreads();
-        for (int t = 0; t < kSzChunk; t++) {
+        for (int t = 0; t < kSpChunk; t++) {
             const int n = n0 + t;
             if (n >= m) break;
-            const double w = inv[n];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const double fp = tile[q][lane][t];  // node n+1 (not used at n = m-1)
-                const double hi = n == m - 1 ? f0[q] : fp;
-                const double lo = n == 0 ? f0[q] : fm[q];
-                y[q] = (r3h * (hi - lo) - y[q]) * w;
-                tile[q][lane][t] = y[q];
-                fm[q] = f0[q];
-                f0[q] = fp;
-            }
+            const double fp = tile[q][lane][t];  // node n+1 (not used at n = m-1)
+            const double hi = n == m - 1 ? f0 : fp;
+            const double lo = n == 0 ? f0 : fm;
+            y = (r3h * (hi - lo) - y) * inv[n];
+            tile[q][lane][t] = y;
+            fm = f0;
+            f0 = fp;
         }
         __syncthreads();
         coop(n0, [&](int l, int t, unsigned c) {
@@ -407,24 +435,20 @@ __global__ void __launch_bounds__(kSzLines) k_spline_z(Geo g, St st, const doubl
         __syncthreads();
     }
     // backward from D_{m-1} = y_{m-1}: D_n = y_n - inv_n D_{n+1}
-    double d[3] = {y[0], y[1], y[2]};
+    double d = y;
     for (int ch = nch - 1; ch >= 0; ch--) {
-        const int n0 = ch * kSzChunk;
+        const int n0 = ch * kSpChunk;
         coop(n0, [&](int l, int t, unsigned c) {
             tile[0][l][t] = ld(G[0], c);
             tile[1][l][t] = ld(G[1], c);
             tile[2][l][t] = ld(G[2], c);
         });
         __syncthreads();
-        for (int t = kSzChunk - 1; t >= 0; t--) {
+        for (int t = kSpChunk - 1; t >= 0; t--) {
             const int n = n0 + t;
             if (n >= m - 1) continue;  // node m-1 keeps y_{m-1}
-            const double w = inv[n];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                d[q] = tile[q][lane][t] - w * d[q];
-                tile[q][lane][t] = d[q];
-            }
+            d = tile[q][lane][t] - inv[n] * d;
+            tile[q][lane][t] = d;
         }
         __syncthreads();
         coop(n0, [&](int l, int t, unsigned c) {
@@ -436,81 +460,291 @@ __global__ void __launch_bounds__(kSzLines) k_spline_z(Geo g, St st, const doubl
     }
 }
 
-// PrintStats sums over the interior (update.cpp:658-707):
-//  0 rho|u|^2  1 mu du_i/dx_j du_i/dx_j  2 S_ij S_ij  3..5 (du/dx)^2,3,4
-//  6..8 (dv/dy)^n  9..11 (dw/dz)^n  12 p  13 T  14 1/rho
+// chunked pass over the lines of axis A: blocks of 64 lines x 3 components
+template <class St, int A>
+__global__ void __launch_bounds__(kSpThreads) k_spl_chunked(Geo g, St st, const double* __restrict__ inv,
+                                                            double r3h, double* __restrict__ grad) {
+    if constexpr (A == 2) {
+        __shared__ double tile[3][kSpLines][kSpChunk + 1];
+        spline_z_block(g, st, inv, r3h, grad, (int)blockIdx.x, tile);
+    } else {
+        spline_strided_block<St, A>(g, st, inv, r3h, grad, (int)blockIdx.x);
+    }
+}
+
+// ---- whole lines in LDS ------------------------------------------------------
+// For lines of up to kSplMaxM nodes (boxes up to 335 cells a side) a block
+// holds a group of kSplL whole lines, all three components, in LDS:
+//   load:  every thread loads state and forms u, v, w at the group's nodes
+//   walk:  lane (component c, line l), 24 lanes, runs the forward and the
+//          backward recursion in place (f -> y -> D), out of LDS
+//   out:   the interior slopes go to the axis's planes.
+// Nothing but the state and the slopes crosses HBM (the chunked kernels
+// above also store and reload y), and every load and store covers a group's
+// lines at a node: 8 consecutive z for x and y lines, and runs along the
+// line for z lines, which are contiguous.  Measured at 256^3 (one MI355X):
+// about 370 us per axis against 450-580 us for the chunked passes.
+constexpr int kSplL = 8, kSplThreads = 192;
+constexpr int kSplMaxM = 341;  // 3 kSplL m doubles within 64 KB
+
+// line t of axis A (lines over the two other axes' interiors, the faster inner)
+template <int A> struct SplAxis {
+    unsigned n_in, n_out, stride;
+    int m;
+    __device__ SplAxis(const Geo& g)
+        : n_in(A == 2 ? g.nx : g.nz), n_out(A == 0 ? g.nx : g.ny),
+          stride(A == 0 ? g.s0 : (A == 1 ? g.s1 : 1u)), m(A == 0 ? g.P0 : (A == 1 ? g.P1 : g.P2)) {}
+    __device__ unsigned base(const Geo& g, unsigned t) const {
+        const unsigned a = t % n_in + 3, b = t / n_in + 3;
+        return A == 0 ? b * g.s1 + a : (A == 1 ? b * g.s0 + a : b * g.s0 + a * g.s1);
+    }
+};
+
+// Elements (line l, node n) of a group as the 192 threads share them: for x
+// and y lines thread t keeps line t % 8 and takes nodes t / 8 + 24 k (a
+// node's 8 lines are 8 consecutive z); for z lines, contiguous, thread t
+// takes node t + 192 k of all 8 lines.
+//
+// spl_load: the velocities of the group's nodes into the tile.  A batch's
+// loads (8 elements) are all issued before its divisions and LDS writes, with
+// no branch around an element: a missing line (the last group's) repeats the
+// group's last line, a node past m - 1 repeats node m - 1; their slots get the
+// same values again or are never read.
+template <class St, int A>
+__device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAxis<A>& ax, unsigned t0,
+                                         unsigned nl, double* tile) {
+    const int m = ax.m, t = threadIdx.x;
+    constexpr int K = 8;
+    // the batch's raw words (the mixed state's hi and lo floats) are loaded
+    // into registers first and only then combined: combining each value as
+    // it is loaded had the compiler wait for every pair of loads in turn
+    using Raw = typename std::conditional<std::is_same<St, StateM>::value, float, double>::type;
+    auto raw = [&](int q, unsigned c, Raw& h, Raw& l) {
+        if constexpr (std::is_same<St, StateM>::value) {
+            h = ld(st.hi[q], c);
+            l = ld(st.lo[q], c);
+        } else {
+            h = ld(st.w[q], c);
+            l = Raw(0);
+        }
+    };
+    auto put_f = [&](int l, int n, const double (&v)[4]) {
+        tile[(0 * m + n) * kSplL + l] = v[1] / v[0];
+        tile[(1 * m + n) * kSplL + l] = v[2] / v[0];
+        tile[(2 * m + n) * kSplL + l] = v[3] / v[0];
+    };
+    if (A == 2) {
+        static_assert(K == kSplL, "one batch = one node of the 8 lines");
+        unsigned b[kSplL];
+#pragma unroll
+        for (int l = 0; l < kSplL; l++) b[l] = ax.base(g, min(t0 + l, nl - 1));
+        for (int n = t; n < m; n += kSplThreads) {
+            Raw h[kSplL][4], lo[kSplL][4];
+#pragma unroll
+            for (int l = 0; l < kSplL; l++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) raw(q, b[l] + (unsigned)n, h[l][q], lo[l][q]);
+#pragma unroll
+            for (int l = 0; l < kSplL; l++) {
+                double v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = (double)h[l][q] + (double)lo[l][q];
+                put_f(l, n, v);
+            }
+        }
+    } else {
+        const int l = t % kSplL, step = kSplThreads / kSplL;
+        const unsigned b = ax.base(g, min(t0 + l, nl - 1));
+        for (int n = t / kSplL; n < m; n += K * step) {
+            Raw h[K][4], lo[K][4];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const unsigned c = b + (unsigned)min(n + k * step, m - 1) * ax.stride;
+#pragma unroll
+                for (int q = 0; q < 4; q++) raw(q, c, h[k][q], lo[k][q]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                double v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = (double)h[k][q] + (double)lo[k][q];
+                put_f(l, min(n + k * step, m - 1), v);
+            }
+        }
+    }
+}
+
+// spl_store: body(l, n, c) for the group's existing lines, nodes [n0, n1)
+template <int A, class F>
+__device__ __forceinline__ void spl_store(const Geo& g, const SplAxis<A>& ax, unsigned t0, unsigned nl, int n0,
+                                          int n1, F&& body) {
+    const int t = threadIdx.x;
+    if (A == 2) {
+        for (int l = 0; l < kSplL; l++) {
+            if (t0 + l >= nl) break;
+            const unsigned b = ax.base(g, t0 + l);
+            for (int n = n0 + t; n < n1; n += kSplThreads) body(l, n, b + (unsigned)n);
+        }
+    } else {
+        const int l = t % kSplL;
+        if (t0 + l >= nl) return;
+        const unsigned b = ax.base(g, t0 + l);
+        for (int n = n0 + t / kSplL; n < n1; n += kSplThreads / kSplL) body(l, n, b + (unsigned)n * ax.stride);
+    }
+}
+
+// The group's velocities into tile[(c m + n) kSplL + l] (c = component),
+// then lane (c, l) of the first 24 solves its line in place: f -> y -> D.
+template <class St, int A>
+__device__ __forceinline__ void spl_group_solve(const Geo& g, const St& st, const SplAxis<A>& ax, unsigned t0,
+                                                unsigned nl, const double* __restrict__ inv, double r3h,
+                                                double* tile) {
+    const int m = ax.m;
+    spl_load(g, st, ax, t0, nl, tile);
+    __syncthreads();
+    if (threadIdx.x < 3 * kSplL) {
+        const int c = threadIdx.x / kSplL, l = threadIdx.x % kSplL;
+        double* T = tile + (size_t)c * m * kSplL + l;  // node n at T[n * kSplL]
+        constexpr int U = 8;  // LDS reads of U nodes ahead of the dependent steps
+        // forward: node 0 (one-sided), 1 .. m-2 (central), m-1 (one-sided)
+        double fm = T[0], f0 = T[kSplL];
+        double y = (r3h * (f0 - fm)) * inv[0];  // y_{-1} = 0
+        T[0] = y;
+        int i = 1;
+        for (; i + U <= m - 1; i += U) {  // f of nodes i+1 .. i+U, all <= m-1
+            double fn[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) fn[k] = T[(i + 1 + k) * kSplL];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                y = (r3h * (fn[k] - fm) - y) * inv[i + k];
+                T[(i + k) * kSplL] = y;
+                fm = f0;
+                f0 = fn[k];
+            }
+        }
+        for (; i <= m - 2; i++) {
+            const double fp = T[(i + 1) * kSplL];
+            y = (r3h * (fp - fm) - y) * inv[i];
+            T[i * kSplL] = y;
+            fm = f0;
+            f0 = fp;
+        }
+        y = (r3h * (f0 - fm) - y) * inv[m - 1];
+        T[(m - 1) * kSplL] = y;
+        // backward from D_{m-1} = y_{m-1} down to node 3
+        double d = y;
+        int n = m - 2;
+        for (; n - (U - 1) >= 3; n -= U) {
+            double yv[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) yv[k] = T[(n - k) * kSplL];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                d = yv[k] - inv[n - k] * d;
+                T[(n - k) * kSplL] = d;
+            }
+        }
+        for (; n >= 3; n--) {
+            d = T[n * kSplL] - inv[n] * d;
+            T[n * kSplL] = d;
+        }
+    }
+    __syncthreads();
+}
+
+// lines of axis A: interior slopes into planes 3 A .. 3 A + 2
+// (LDS limits a CU to three blocks, at most three wavefronts per SIMD; saying so
+// lets the compiler keep a whole batch of loads in flight instead of budgeting
+// registers for eight wavefronts)
+template <class St, int A>
+__global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_eu(1, 3)))
+    k_spl_lines(Geo g, St st, const double* __restrict__ inv,
+                                                           double r3h, double* __restrict__ grad) {
+    extern __shared__ double spl_tile[];
+    const SplAxis<A> ax(g);
+    const unsigned nl = ax.n_in * ax.n_out;
+    const unsigned t0 = xcd_block() * kSplL;
+    if (t0 >= nl) return;  // block-uniform
+    spl_group_solve(g, st, ax, t0, nl, inv, r3h, spl_tile);
+    const int m = ax.m;
+    spl_store(g, ax, t0, nl, 3, m - 3, [&](int l, int n, unsigned c) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) put(grad + (size_t)(3 * A + q) * g.ncell, c, spl_tile[(q * m + n) * kSplL + l]);
+    });
+}
+
+// The sums over the interior from the nine slope planes and the state,
+// grid-stride over the interior cells (z fastest); one partial row per block.
 template <class St>
 __global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const double* __restrict__ grad, StatPar p,
-                                                        double* part) {
+                                                        FastDiv fnz, FastDiv fnx, double* part) {
     double acc[kPStat];
 #pragma unroll
     for (int q = 0; q < kPStat; q++) acc[q] = 0.0;
-    const long n = (long)g.nx * g.ny * g.nz;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(t % g.nz);
-        const long r = t / g.nz;
-        const int i = (int)(r % g.nx);
-        const int j = (int)(r / g.nx);
-        const unsigned c = (unsigned)(j + 3) * g.s0 + (unsigned)(i + 3) * g.s1 + (unsigned)(k + 3);
-        const double rho = st.get(0, c);
-        double u[3];
-        vel(st, c, u);
-        const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-        const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
-        const double T = P / (p.R * rho);
-        auto gr = [&](int a, int comp) { return ld(grad + (size_t)(3 * a + comp) * g.ncell, c); };
-        const double dudx = gr(1, 0), dudy = gr(0, 0), dudz = gr(2, 0);
-        const double dvdx = gr(1, 1), dvdy = gr(0, 1), dvdz = gr(2, 1);
-        const double dwdx = gr(1, 2), dwdy = gr(0, 2), dwdz = gr(2, 2);
-        const double mu = visc(p, T);
-        acc[0] += rho * q2;
-        acc[1] += mu * (dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
-                        dwdx * dwdx + dwdy * dwdy + dwdz * dwdz);
-        acc[2] += dudx * dudx + 0.5 * (dudy + dvdx) * (dudy + dvdx) + 0.5 * (dudz + dwdx) * (dudz + dwdx) +
-                  0.5 * (dwdy + dvdz) * (dwdy + dvdz) + dvdy * dvdy + dwdz * dwdz;
-        double x2 = dudx * dudx; acc[3] += x2; acc[4] += x2 * dudx; acc[5] += x2 * x2;
-        x2 = dvdy * dvdy; acc[6] += x2; acc[7] += x2 * dvdy; acc[8] += x2 * x2;
-        x2 = dwdz * dwdz; acc[9] += x2; acc[10] += x2 * dwdz; acc[11] += x2 * x2;
-        acc[12] += P; acc[13] += T; acc[14] += 1.0 / rho;
+    const unsigned n = (unsigned)g.nx * g.ny * g.nz;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const unsigned r = fdiv(t, fnz), k = t - r * fnz.d;
+        const unsigned j = fdiv(r, fnx), i = r - j * fnx.d;
+        const unsigned c = (j + 3) * g.s0 + (i + 3) * g.s1 + (k + 3);
+        double dX[3], dY[3], dZ[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            dY[q] = ld(grad + (size_t)q * g.ncell, c);
+            dX[q] = ld(grad + (size_t)(3 + q) * g.ncell, c);
+            dZ[q] = ld(grad + (size_t)(6 + q) * g.ncell, c);
+        }
+        pstat_add(st, c, p, dX, dY, dZ, acc);
     }
     block_reduce_store<kPStat, 0u>(acc, part);
 }
 
+// The slopes of the three axes into their planes (whole lines in LDS where
+// they fit, m <= kSplMaxM, else in chunks), then the sums; returns the number
+// of partial rows.
 template <class St>
-static void launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
+static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
     const Geo& g = h->geo;
     const double r3h = 3.0 / h->grid.dx;
-    const double* inv0 = h->spl_inv;
-    const double* inv1 = inv0 + g.P0;
-    const double* inv2 = inv1 + g.P1;
-    const unsigned ly = (unsigned)g.nx * g.nz, lx = (unsigned)g.ny * g.nz, lz = (unsigned)g.ny * g.nx;
-    hipLaunchKernelGGL((k_spline<St, 0>), dim3((ly + 255) / 256), dim3(256), 0, h->stream, g, st, inv0, r3h,
-                       h->spl_grad);
-    hipLaunchKernelGGL((k_spline<St, 1>), dim3((lx + 255) / 256), dim3(256), 0, h->stream, g, st, inv1, r3h,
-                       h->spl_grad);
-    hipLaunchKernelGGL(k_spline_z<St>, dim3((lz + kSzLines - 1) / kSzLines), dim3(kSzLines), 0, h->stream, g, st,
-                       inv2, r3h, h->spl_grad);
+    const double* inv[3] = {h->spl_inv, h->spl_inv + g.P0, h->spl_inv + g.P0 + g.P1};
+    const int m[3] = {g.P0, g.P1, g.P2};
+    const unsigned nl[3] = {(unsigned)g.nx * g.nz, (unsigned)g.ny * g.nz, (unsigned)g.ny * g.nx};
+    auto tile_bytes = [](int mm) { return sizeof(double) * 3 * (size_t)mm * kSplL; };
+    auto axis = [&](auto Ax) {
+        constexpr int A = decltype(Ax)::value;
+        if (m[A] <= kSplMaxM)
+            hipLaunchKernelGGL((k_spl_lines<St, A>), dim3((nl[A] + kSplL - 1) / kSplL), dim3(kSplThreads),
+                               tile_bytes(m[A]), h->stream, g, st, inv[A], r3h, h->spl_grad);
+        else
+            hipLaunchKernelGGL((k_spl_chunked<St, A>), dim3((nl[A] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0,
+                               h->stream, g, st, inv[A], r3h, h->spl_grad);
+    };
+    axis(std::integral_constant<int, 2>());
+    axis(std::integral_constant<int, 1>());
+    axis(std::integral_constant<int, 0>());
     hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, h->stream, g, st, h->spl_grad, p,
-                       h->red_part);
+                       make_fastdiv((unsigned)g.nz), make_fastdiv((unsigned)g.nx), h->red_part);
+    return h->red_blocks;
 }
 
 hipError_t launch_print_stats(Ctx* h, int box, double* host_out) {
     const StatPar p = stat_par(h, box);
     const size_t off = (size_t)box * h->ncell;
+    int nb;
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
-        launch_print_stats_st(h, st, p);
+        nb = launch_print_stats_st(h, st, p);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q] + off;
             st.lo[q] = h->Wlo[q] + off;
         }
-        launch_print_stats_st(h, st, p);
+        nb = launch_print_stats_st(h, st, p);
     }
-    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, h->red_blocks,
-                       h->red_part, h->red_out);
+    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, nb, h->red_part,
+                       h->red_out);
     hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kPStat, hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(h->stream);

@@ -301,16 +301,6 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 // needs 260 and runs at 1 wave, about 1.5x slower.  FP32 fits 3 waves without
 // a bound; forcing 4 spills.
 // ---------------------------------------------------------------------------
-// Blocks are dealt to the 8 XCDs round-robin by blockIdx.x.  The logical block
-// number below gives XCD x the x-th contiguous eighth of the blocks, so that
-// neighbouring rows of interfaces, which read each other's interface values,
-// run on one XCD and share its L2.  A bijection on [0, gridDim.x).  (For a
-// batch, blockIdx.y > 0 rows start on other XCDs when gridDim.x is not a
-// multiple of 8; the mapping stays a bijection, only the locality is lost.)
-__device__ __forceinline__ unsigned xcd_block() {
-    const unsigned nb = gridDim.x, xcd = blockIdx.x & 7u, q = nb >> 3, rem = nb & 7u;
-    return xcd * q + min(xcd, rem) + (blockIdx.x >> 3);
-}
 
 template <class Real, class St, int DIR>
 __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, unsigned st2, const St& st,
@@ -439,39 +429,59 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
     hi = h2;
 }
 
-// new value of interior cell c into the other buffer: x and y flux planes of
-// the earlier sweeps, z fluxes of the cell's right (Fz) and left (Fm) faces.
-// Every load comes before the first store (the planes may alias as far as the
-// compiler knows, and interleaving would serialise the fields' round trips).
-__device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateD& S, const StateD& N,
-                                            const Planes15<double>& F, const double Fz[5], const double Fm[5],
-                                            const StepPar& par) {
-    double d[5], w[5];
+// The update of interior cell c into the other buffer, in two halves around
+// the z flux.  update_stage loads what does not depend on the z flux -- the x
+// and y flux differences (summed in the reference's order as far as they go)
+// and the old state -- and parks it in LDS, slot s of lane t at
+// lds[s * blockDim.x + t]: issued before the flux, these loads overlap the
+// flux's own input loads instead of stalling the wave after it, and they hold
+// no registers across the flux.  update_finish adds the two z fluxes of the
+// cell's right (Fz) and left (Fm) faces and stores the new value; the sum is
+// the same expression, term for term, as in one pass.
+template <class St> struct UpdSlots;
+template <> struct UpdSlots<StateD> { static constexpr int n = 10; };  // d[5], w[5]
+template <> struct UpdSlots<StateM> { static constexpr int n = 15; };  // R[5], hi[5], lo[5]
+
+__device__ __forceinline__ void update_stage(const Geo& g, unsigned c, const StateD& S, const Planes15<double>& F,
+                                             double* lds) {
+    const unsigned t = threadIdx.x, b = blockDim.x;
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        d[q] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0) + Fz[q] - Fm[q];
-        w[q] = ld(S.w[q], c);
+        lds[q * b + t] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0);
+        lds[(5 + q) * b + t] = ld(S.w[q], c);
     }
+}
+__device__ __forceinline__ void update_finish(unsigned c, const StateD& N, const double Fz[5], const double Fm[5],
+                                              const double* lds, const StepPar& par) {
+    const unsigned t = threadIdx.x, b = blockDim.x;
     const double idx = par.ud[kUIdx];  // 1 / dx
 #pragma unroll
-    for (int q = 0; q < 5; q++) put(N.w[q], c, w[q] - idx * d[q]);
+    for (int q = 0; q < 5; q++) {
+        const double d = lds[q * b + t] + Fz[q] - Fm[q];
+        put(N.w[q], c, lds[(5 + q) * b + t] - idx * d);
+    }
 }
-__device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const StateM& S, const StateM& N,
-                                            const Planes15<float>& F, const float Fz[5], const float Fm[5],
-                                            const StepPar& par) {
-    float R[5], hi[5], lo[5];
+__device__ __forceinline__ void update_stage(const Geo& g, unsigned c, const StateM& S, const Planes15<float>& F,
+                                             float* lds) {
+    const unsigned t = threadIdx.x, b = blockDim.x;
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        R[q] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0) + Fz[q] - Fm[q];
-        hi[q] = ld(S.hi[q], c);
-        lo[q] = ld(S.lo[q], c);
+        lds[q * b + t] = ld(F.p[q], c) - ld(F.p[q], c - g.s1) + ld(F.p[5 + q], c) - ld(F.p[5 + q], c - g.s0);
+        lds[(5 + q) * b + t] = ld(S.hi[q], c);
+        lds[(10 + q) * b + t] = ld(S.lo[q], c);
     }
+}
+__device__ __forceinline__ void update_finish(unsigned c, const StateM& N, const float Fz[5], const float Fm[5],
+                                              const float* lds, const StepPar& par) {
+    const unsigned t = threadIdx.x, b = blockDim.x;
     const float idx = par.uf[kUIdx];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        two_sum_add(hi[q], lo[q], -idx * R[q]);
-        put(N.hi[q], c, hi[q]);
-        put(N.lo[q], c, lo[q]);
+        const float R = lds[q * b + t] + Fz[q] - Fm[q];
+        float hi = lds[(5 + q) * b + t], lo = lds[(10 + q) * b + t];
+        two_sum_add(hi, lo, -idx * R);
+        put(N.hi[q], c, hi);
+        put(N.lo[q], c, lo);
     }
 }
 
@@ -483,7 +493,8 @@ __device__ __forceinline__ void update_cell(const Geo& g, unsigned c, const Stat
 // wavefronts share one: lane l computes the flux at its interface, takes its
 // left neighbour's from lane l-1 (__shfl_up), and for l >= 1 and k >= 3
 // updates cell k from the x, y fluxes of the earlier sweeps and the two z
-// fluxes (update_cell).  The state is read from the current buffer and the
+// fluxes (update_stage before the flux, update_finish after it).  The state
+// is read from the current buffer and the
 // new interior written to the other one, so no lane's read of a neighbour
 // cell can see an updated value.  The flux is the WENO-BGK one (flux_at, from
 // the interface planes) or the GKS one (gks_at, from the cells).
@@ -534,8 +545,19 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     // flux so that no per-lane index value stays live in a vector register
     // across it
     const unsigned long long upd_mask = __ballot(upd);
+    extern __shared__ unsigned char upd_lds_raw[];  // UpdSlots<St>::n * blockDim.x values
+    Real* upd_lds = reinterpret_cast<Real*>(upd_lds_raw);
+    // Where the staging goes was measured (256^3): FP64, in a block of its own
+    // ahead of the flux (fused kernel 2.05 -> 1.91 ms; the compiler issues it
+    // first and waits for it before the flux's loads); FP32, in the flux's block
+    // (0.875 -> 0.83 ms; its loads are scheduled among the flux's own).  Every
+    // valid lane stages in the second form: c - s0 and c - s1 are in the box
+    // for k = 2 and for lane 0 too, and their slots are never read.
+    constexpr bool kStageFirst = sizeof(Real) == 8;
+    if (kStageFirst && upd) update_stage(g, c, st, F, upd_lds);
     Real Fz[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
     if (valid) {
+        if (!kStageFirst) update_stage(g, c, st, F, upd_lds);
         Real Fs[5];
         if constexpr (SCHEME == GKM_SCHEME_GKS)
             gks_at<Real, St, 2>(c, 1u, st1, st2, st, par, Fs);
@@ -548,7 +570,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
 #pragma unroll
     for (int q = 0; q < 5; q++) Fm[q] = __shfl_up(Fz[q], 1u, 64);
     if (!((upd_mask >> __lane_id()) & 1ull)) return;
-    update_cell(g, c, st, nx, F, Fz, Fm, par);
+    update_finish(c, nx, Fz, Fm, upd_lds, par);
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -786,15 +808,16 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
     const int bs = h->flux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
+    const size_t lds = sizeof(Real) * UpdSlots<St>::n * (size_t)bs;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction
         if (ev) (void)hipEventRecord(ev[1], h->stream);
-        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_GKS>), grid, dim3(bs), 0, h->stream, g, zl, st1, st2,
+        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_GKS>), grid, dim3(bs), lds, h->stream, g, zl, st1, st2,
                            st, nx, tau, wi, F, h->d_pars);
     } else {
         launch_recon_any<Real, St>(h, 2, st, Tw, wi);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
-        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_WENO_BGK>), grid, dim3(bs), 0, h->stream, g, zl, st1,
+        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_WENO_BGK>), grid, dim3(bs), lds, h->stream, g, zl, st1,
                            st2, st, nx, tau, wi, F, h->d_pars);
     }
     if (ev) (void)hipEventRecord(ev[2], h->stream);

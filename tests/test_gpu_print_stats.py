@@ -70,6 +70,22 @@ def test_print_stats_fp64(gk, orc, shape, law):
     _check(dev, ref, 1e-11)
 
 
+@pytest.mark.parametrize("shape", [(335, 6, 7), (336, 6, 7), (5, 340, 4), (4, 5, 340)])
+def test_print_stats_long_lines(gk, orc, shape):
+    """Lines of up to 341 padded nodes (335 cells) are solved whole in LDS,
+    longer ones in 16-node chunks through the gradient planes: one axis at a
+    time on either side of that limit, against the oracle."""
+    nx, ny, nz = shape
+    print("print_stats long lines", shape, flush=True)
+    grid, W = _rough_state(orc, nx, ny, nz, seed=nx + 7 * nz)
+    gas = _gas(gk)
+    with gk.Solver(nx, ny, nz, gas=gas) as s:
+        s.set_state(W)
+        dev = s.print_stats()
+    ref = orc.print_stats(grid, _orc_params(orc, gas), W)
+    _check(dev, ref, 1e-11)
+
+
 def test_print_stats_mixed(gk, orc):
     nx, ny, nz = 16, 12, 20
     print("print_stats mixed", (nx, ny, nz), flush=True)
