@@ -161,8 +161,8 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     auto up = [&](size_t b) { return (b + align - 1) / align * align; };
     const bool fp64 = params[0].precision == GKM_PREC_FP64;
     const size_t plane = up(n * (fp64 ? sizeof(double) : sizeof(float)));
-    // state (x2: current and next), interface values, fluxes, T and tau
-    const int nplanes = fp64 ? (10 + 10 + 15 + 2) : (20 + 10 + 15 + 2);
+    // state (x2: current and next), interface values, fluxes, T and tau (x2)
+    const int nplanes = fp64 ? (10 + 10 + 15 + 3) : (20 + 10 + 15 + 3);
     h->red_blocks = 2048;
     const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
     const size_t pars = up(sizeof(gkm::StepPar) * nbox);
@@ -187,6 +187,7 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
         for (int q = 0; q < 15; q++) h->Fd[q] = (double*)take(plane);
         h->Td = (double*)take(plane);
         h->taud = (double*)take(plane);
+        h->taud_nx = (double*)take(plane);
     } else {
         for (int q = 0; q < 5; q++) h->Whi[q] = (float*)take(plane);
         for (int q = 0; q < 5; q++) h->Wlo[q] = (float*)take(plane);
@@ -196,6 +197,7 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
         for (int q = 0; q < 15; q++) h->Ff[q] = (float*)take(plane);
         h->Tf = (float*)take(plane);
         h->tauf = (float*)take(plane);
+        h->tauf_nx = (float*)take(plane);
     }
     h->red_part = (double*)take(up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA));
     h->red_out = (double*)take(up(sizeof(double) * 64));
@@ -248,6 +250,7 @@ int gkm_set_params_box(gkm_t* hh, int box, const gkm_params* p) {
     h->bpar[box] = *p;
     if (box == 0) h->par = *p;
     h->pars_valid = false;
+    h->derived_valid = false;  // Tw and tau depend on the gas constants
     return GKM_OK;
 }
 
@@ -278,6 +281,7 @@ int gkm_set_state_box(gkm_t* hh, int box, const double* const W[5]) {
         if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
     const size_t bytes = sizeof(double) * (size_t)h->ncell;
     const size_t off = (size_t)box * h->ncell;
+    h->derived_valid = false;
     if (h->par.precision == GKM_PREC_FP64) {
         for (int q = 0; q < 5; q++)
             HIP_TRY(hipMemcpyAsync(h->Wd[q] + off, W[q], bytes, hipMemcpyHostToDevice, h->stream));
@@ -347,7 +351,10 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
             h->timed_steps = s + 1;
         } else if (h->graph_enabled) {
             // a captured step holds the buffer pointers it was captured with:
-            // one graph per starting buffer
+            // one graph per starting buffer.  The ghost fill a stale state
+            // needs runs before, outside the graph, so that the graph holds
+            // the same work on every replay.
+            gkm::prepare_step(h);
             const int f = h->flip;
             const bool capture = !h->graph_ready[f];
             if (capture) {
@@ -365,7 +372,7 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
             gkm::launch_step(h, nullptr);
         }
     }
-    gkm::launch_ghost(h, false);
+    gkm::ensure_ghosts(h);
     HIP_TRY(hipGetLastError());
     return GKM_OK;
 }
@@ -503,13 +510,15 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
             int rc = upload_pars(h, dt.data());
             if (rc) return rc;
         }
+        gkm::prepare_step(h);  // outside the graph, as for a single handle
         // the group's work follows what is queued on the handle
         HIP_TRY(hipEventRecord(g->pre[i], h->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, g->pre[i], 0));
     }
     for (int s = 0; s < nsteps; s++) {
+        // a capture is keyed by each handle's buffer parity and halo mode
         std::vector<int> flips(n);
-        for (int i = 0; i < n; i++) flips[i] = g->hs[i]->flip;
+        for (int i = 0; i < n; i++) flips[i] = g->hs[i]->flip + 2 * g->hs[i]->halo_mode;
         gkm_group::Cap* cap = nullptr;
         for (auto& c : g->caps)
             if (c.flips == flips) cap = &c;
@@ -543,7 +552,7 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
     HIP_TRY(hipEventRecord(g->done, g->stream));
     for (Ctx* h : g->hs) {
         HIP_TRY(hipStreamWaitEvent(h->stream, g->done, 0));
-        gkm::launch_ghost(h, false);
+        gkm::ensure_ghosts(h);
     }
     HIP_TRY(hipGetLastError());
     return GKM_OK;
@@ -634,6 +643,7 @@ int gkm_set_halo_mode(gkm_t* hh, int mode) {
         HIP_TRY(hipMalloc(&h->halo_recv, bytes));
     }
     h->halo_mode = mode;
+    h->derived_valid = false;
     drop_graph(h);
     return GKM_OK;
 }
@@ -659,7 +669,7 @@ int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
     int rc = check_box(h, box);
     if (rc) return rc;
     if (!out) return fail(GKM_E_ARG, "null argument");
-    gkm::launch_ghost(h, false);
+    gkm::ensure_ghosts(h);
     double a[gkm::kStatA];
     HIP_TRY(gkm::launch_stats_a(h, box, a));
     const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
@@ -736,7 +746,7 @@ int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
         HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
         HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     }
-    gkm::launch_ghost(h, false);
+    gkm::ensure_ghosts(h);
     double a[gkm::kPStat];
     HIP_TRY(gkm::launch_print_stats(h, box, a));
     const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;

@@ -49,6 +49,7 @@ struct Ctx {
     double* Fd[15] = {};
     double* Td = nullptr;    // WENO temperature input per cell
     double* taud = nullptr;  // left-cell collision time mu/p per cell
+    double* taud_nx = nullptr;  // the other buffer's (the Z sweep reads taud while writing it)
     // mixed form
     float* Whi[5] = {};
     float* Wlo[5] = {};
@@ -56,6 +57,12 @@ struct Ctx {
     float* Ff[15] = {};
     float* Tf = nullptr;
     float* tauf = nullptr;
+    float* tauf_nx = nullptr;
+    // the current buffer's ghost cells, and for the WENO-BGK scheme its Tw and
+    // tau planes, are those of its interior (written by the last step's Z sweep
+    // or by a ghost fill); anything that changes the state or the gas
+    // constants clears it
+    bool derived_valid = false;
     // one device allocation holds every plane
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -83,6 +90,8 @@ int report_error(int code, const char* msg);
 
 StepPar step_par(const gkm_grid& grid, const gkm_params& p, double dt);
 void launch_ghost(Ctx* h, bool prim);  // prim = false: ghosts only
+void prepare_step(Ctx* h);              // ghosts and primitives valid before a step (periodic fill)
+void ensure_ghosts(Ctx* h);             // ghosts valid for a read of the state
 // one step; for the WENO-BGK scheme it leaves the new state in the other
 // buffer and swaps the current-state pointers (swap_state)
 void launch_step(Ctx* h, hipEvent_t* ev);  // ev: kEvPerStep events or nullptr

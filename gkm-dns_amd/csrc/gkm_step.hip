@@ -438,6 +438,28 @@ __device__ __forceinline__ void two_sum_add(float& hi, float& lo, float d) {
 // no registers across the flux.  update_finish adds the two z fluxes of the
 // cell's right (Fz) and left (Fm) faces and stores the new value; the sum is
 // the same expression, term for term, as in one pass.
+// new value of a cell in the state's own representation
+template <class St> struct CellVal;
+template <> struct CellVal<StateD> {
+    double w[5];
+    __device__ __forceinline__ double get(int q) const { return w[q]; }
+};
+template <> struct CellVal<StateM> {
+    float hi[5], lo[5];
+    __device__ __forceinline__ double get(int q) const { return (double)hi[q] + (double)lo[q]; }
+};
+__device__ __forceinline__ void store_cell(const StateD& N, unsigned c, const CellVal<StateD>& v) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) put(N.w[q], c, v.w[q]);
+}
+__device__ __forceinline__ void store_cell(const StateM& N, unsigned c, const CellVal<StateM>& v) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        put(N.hi[q], c, v.hi[q]);
+        put(N.lo[q], c, v.lo[q]);
+    }
+}
+
 template <class St> struct UpdSlots;
 template <> struct UpdSlots<StateD> { static constexpr int n = 10; };  // d[5], w[5]
 template <> struct UpdSlots<StateM> { static constexpr int n = 15; };  // R[5], hi[5], lo[5]
@@ -452,14 +474,15 @@ __device__ __forceinline__ void update_stage(const Geo& g, unsigned c, const Sta
     }
 }
 __device__ __forceinline__ void update_finish(unsigned c, const StateD& N, const double Fz[5], const double Fm[5],
-                                              const double* lds, const StepPar& par) {
+                                              const double* lds, const StepPar& par, CellVal<StateD>& v) {
     const unsigned t = threadIdx.x, b = blockDim.x;
     const double idx = par.ud[kUIdx];  // 1 / dx
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double d = lds[q * b + t] + Fz[q] - Fm[q];
-        put(N.w[q], c, lds[(5 + q) * b + t] - idx * d);
+        v.w[q] = lds[(5 + q) * b + t] - idx * d;
     }
+    store_cell(N, c, v);
 }
 __device__ __forceinline__ void update_stage(const Geo& g, unsigned c, const StateM& S, const Planes15<float>& F,
                                              float* lds) {
@@ -472,7 +495,7 @@ __device__ __forceinline__ void update_stage(const Geo& g, unsigned c, const Sta
     }
 }
 __device__ __forceinline__ void update_finish(unsigned c, const StateM& N, const float Fz[5], const float Fm[5],
-                                              const float* lds, const StepPar& par) {
+                                              const float* lds, const StepPar& par, CellVal<StateM>& v) {
     const unsigned t = threadIdx.x, b = blockDim.x;
     const float idx = par.uf[kUIdx];
 #pragma unroll
@@ -480,8 +503,58 @@ __device__ __forceinline__ void update_finish(unsigned c, const StateM& N, const
         const float R = lds[q * b + t] + Fz[q] - Fm[q];
         float hi = lds[(5 + q) * b + t], lo = lds[(10 + q) * b + t];
         two_sum_add(hi, lo, -idx * R);
-        put(N.hi[q], c, hi);
-        put(N.lo[q], c, lo);
+        v.hi[q] = hi;
+        v.lo[q] = lo;
+    }
+    store_cell(N, c, v);
+}
+
+// The next step's ghost cells and per-cell primitives, from the lane that
+// wrote interior cell (J, I, K) (padded indices): the ghosts whose periodic
+// image the cell is -- J + ny when J < 6, J - ny when J >= ny, the same in x
+// and z, and their combinations for edges and corners (every ghost cell has
+// exactly one interior image, the cell k_ghost_prim copies it from) -- take
+// its new value, and, for the WENO-BGK scheme, the cell and those ghosts take
+// its Tw and tau (cell_prims of the new FP64 state, as k_ghost_prim forms
+// them).  Tw and tau go to the planes the next step reads.
+template <class Real, class St, int SCHEME>
+__device__ __forceinline__ void ghost_images(const Geo& g, unsigned J, unsigned I, unsigned K, unsigned c,
+                                             const St& N, const CellVal<St>& v, Real* Tw, Real* tau,
+                                             const StepPar& par) {
+    constexpr bool prim = SCHEME != GKM_SCHEME_GKS;
+    Real Tc = Real(0), tc = Real(0);
+    if constexpr (prim) {
+        cell_prims<Real>(par, v.get(0), v.get(1), v.get(2), v.get(3), v.get(4), Tc, tc);
+        put(Tw, c, Tc);
+        put(tau, c, tc);
+    }
+    const unsigned n[3] = {(unsigned)g.ny, (unsigned)g.nx, (unsigned)g.nz};
+    const unsigned P[3] = {J, I, K};
+    const int st[3] = {(int)g.s0, (int)g.s1, 1};
+    bool any = false;
+    int off[3][3];  // per axis: own position, low image (+n), high image (-n); 0 = none
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        off[a][0] = 0;
+        off[a][1] = P[a] < 6u ? (int)n[a] * st[a] : 0;
+        off[a][2] = P[a] >= n[a] ? -(int)n[a] * st[a] : 0;
+        any = any || off[a][1] != 0 || off[a][2] != 0;
+    }
+    if (!any) return;
+    for (int x = 0; x < 3; x++) {
+        if (x && !off[0][x]) continue;
+        for (int y = 0; y < 3; y++) {
+            if (y && !off[1][y]) continue;
+            for (int z = 0; z < 3; z++) {
+                if ((z && !off[2][z]) || !(x || y || z)) continue;
+                const unsigned ci = c + (unsigned)(off[0][x] + off[1][y] + off[2][z]);
+                store_cell(N, ci, v);
+                if constexpr (prim) {
+                    put(Tw, ci, Tc);
+                    put(tau, ci, tc);
+                }
+            }
+        }
     }
 }
 
@@ -522,10 +595,11 @@ __device__ __forceinline__ void zline_column(const ZLines& zl, unsigned l, unsig
     i = s * zl.fW.d + (r - j * fw.d);
 }
 
-template <class Real, class St, int SCHEME>
+template <class Real, class St, int SCHEME, bool GHOST>
 __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
-                  Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars) {
+                  Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars, Real* __restrict__ Tw_nx,
+                  Real* __restrict__ tau_nx) {
     const unsigned gt = xcd_block() * blockDim.x + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned e = (gt >> 6) * 63u + lane;
@@ -570,7 +644,19 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
 #pragma unroll
     for (int q = 0; q < 5; q++) Fm[q] = __shfl_up(Fz[q], 1u, 64);
     if (!((upd_mask >> __lane_id()) & 1ull)) return;
-    update_finish(c, nx, Fz, Fm, upd_lds, par);
+    CellVal<St> v;
+    update_finish(c, nx, Fz, Fm, upd_lds, par, v);
+    if constexpr (GHOST) {
+        // the cell's indices again, from the wavefront number and the lane
+        // (written apart from e's expression so that e is not kept live
+        // across the flux)
+        const unsigned e2 = (xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 63u + __lane_id();
+        const unsigned line = fdiv(e2, zl.fnI);
+        const unsigned kk = e2 - line * (zl.fnI.d);
+        unsigned j, i;
+        zline_column(zl, line, j, i);
+        ghost_images<Real, St, SCHEME>(g, j + 3, i + 3, kk + 2, c, nx, v, Tw_nx, tau_nx, par);
+    }
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -788,9 +874,11 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
 
 // Z sweep: reconstruction (WENO-BGK), then the flux fused with the update of
 // every interior cell into the other state buffer
+// ghost: the kernel also writes the next state's ghost cells and its Tw and
+// tau planes (Tw_nx, tau_nx), so the next step needs no ghost fill
 template <class Real, class St>
 static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Real* tau, Planes10<Real> wi,
-                                  Planes15<Real> F, hipEvent_t* ev) {
+                                  Planes15<Real> F, hipEvent_t* ev, bool ghost, Real* Tw_nx, Real* tau_nx) {
     const Geo& g = h->geo;
     int t1, t2;
     tangential_axes(2, h->par.zbug_compat, t1, t2);
@@ -810,26 +898,39 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
     const size_t lds = sizeof(Real) * UpdSlots<St>::n * (size_t)bs;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
+    auto fused = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(bs), lds, h->stream, g, zl, st1, st2, st, nx, tau, wi, F, h->d_pars,
+                           Tw_nx, tau_nx);
+    };
     if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction
         if (ev) (void)hipEventRecord(ev[1], h->stream);
-        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_GKS>), grid, dim3(bs), lds, h->stream, g, zl, st1, st2,
-                           st, nx, tau, wi, F, h->d_pars);
+        if (ghost)
+            fused(k_flux_update<Real, St, GKM_SCHEME_GKS, true>);
+        else
+            fused(k_flux_update<Real, St, GKM_SCHEME_GKS, false>);
     } else {
         launch_recon_any<Real, St>(h, 2, st, Tw, wi);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
-        hipLaunchKernelGGL((k_flux_update<Real, St, GKM_SCHEME_WENO_BGK>), grid, dim3(bs), lds, h->stream, g, zl, st1,
-                           st2, st, nx, tau, wi, F, h->d_pars);
+        if (ghost)
+            fused(k_flux_update<Real, St, GKM_SCHEME_WENO_BGK, true>);
+        else
+            fused(k_flux_update<Real, St, GKM_SCHEME_WENO_BGK, false>);
     }
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
 
+// after a step: the buffers swap, and the new current state's ghosts and
+// primitives are valid when the step's Z sweep wrote them (periodic fill)
 void swap_state(Ctx* h) {
     for (int q = 0; q < 5; q++) {
         std::swap(h->Wd[q], h->Wd_nx[q]);
         std::swap(h->Whi[q], h->Whi_nx[q]);
         std::swap(h->Wlo[q], h->Wlo_nx[q]);
     }
+    std::swap(h->taud, h->taud_nx);
+    std::swap(h->tauf, h->tauf_nx);
     h->flip ^= 1;
+    h->derived_valid = h->halo_mode != GKM_HALO_PACK;
 }
 
 void launch_ghost(Ctx* h, bool prim) {
@@ -913,14 +1014,33 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
     return p;
 }
 
+void prepare_step(Ctx* h) {
+    if (h->halo_mode != GKM_HALO_PACK && !h->derived_valid) {
+        launch_ghost(h, h->par.flux_scheme != GKM_SCHEME_GKS);
+        h->derived_valid = true;
+    }
+}
+
+void ensure_ghosts(Ctx* h) {
+    if (!h->derived_valid) launch_ghost(h, false);
+}
+
 void launch_step(Ctx* h, hipEvent_t* ev) {
     // the GKS scheme reads no per-cell primitive planes; both schemes fuse the
-    // update into their Z sweep, which writes the other state buffer
+    // update into their Z sweep, which writes the other state buffer.  With
+    // the periodic fill the Z sweep also writes that buffer's ghosts and
+    // primitives, and the ghost fill runs only when they are not valid (after
+    // set_state and the like); with the slab exchange (GKM_HALO_PACK) every
+    // step starts with the exchange.
     const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
+    const bool pack = h->halo_mode == GKM_HALO_PACK;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    launch_ghost(h, prim);
+    if (pack)
+        launch_ghost(h, prim);
+    else
+        prepare_step(h);
     if (ev) (void)hipEventRecord(ev[12], h->stream);
-    if (prim && h->halo_mode == GKM_HALO_PACK) launch_prim(h);
+    if (prim && pack) launch_prim(h);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st, nx;
@@ -937,7 +1057,8 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
             for (int q = 0; q < 5; q++) Fd.p[q] = h->Fd[5 * dir + q];
             launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, Fd, ev ? ev + 1 + 3 * dir : nullptr);
         }
-        launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr);
+        launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr, !pack, h->Td,
+                                              h->taud_nx);
     } else {
         StateM st, nx;
         for (int q = 0; q < 5; q++) {
@@ -955,7 +1076,8 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
             for (int q = 0; q < 5; q++) Ff.p[q] = h->Ff[5 * dir + q];
             launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, Ff, ev ? ev + 1 + 3 * dir : nullptr);
         }
-        launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr);
+        launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr, !pack, h->Tf,
+                                             h->tauf_nx);
     }
     if (ev) (void)hipEventRecord(ev[11], h->stream);
     swap_state(h);
