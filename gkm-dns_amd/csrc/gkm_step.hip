@@ -510,17 +510,22 @@ __device__ __forceinline__ void update_finish(unsigned c, const StateM& N, const
 }
 
 // The next step's ghost cells and per-cell primitives, from the lane that
-// wrote interior cell (J, I, K) (padded indices): the ghosts whose periodic
-// image the cell is -- J + ny when J < 6, J - ny when J >= ny, the same in x
-// and z, and their combinations for edges and corners (every ghost cell has
-// exactly one interior image, the cell k_ghost_prim copies it from) -- take
-// its new value, and, for the WENO-BGK scheme, the cell and those ghosts take
-// its Tw and tau (cell_prims of the new FP64 state, as k_ghost_prim forms
-// them).  Tw and tau go to the planes the next step reads.
+// wrote interior cell c: the ghosts whose periodic image the cell is -- padded
+// J + ny when J < 6, J - ny when J >= ny, the same in x and z, and their
+// combinations for edges and corners (every ghost cell has exactly one
+// interior image, the cell k_ghost_prim copies it from) -- take its new value,
+// and, for the WENO-BGK scheme, the cell and those ghosts take its Tw and tau
+// (cell_prims of the new FP64 state, as k_ghost_prim forms them).  Tw and tau
+// go to the planes the next step reads.  imask (image_mask) says which images
+// the cell has.
+__device__ __forceinline__ unsigned image_mask(const Geo& g, unsigned J, unsigned I, unsigned K) {
+    return (J < 6u ? 1u : 0u) | (J >= (unsigned)g.ny ? 2u : 0u) | (I < 6u ? 4u : 0u) |
+           (I >= (unsigned)g.nx ? 8u : 0u) | (K < 6u ? 16u : 0u) | (K >= (unsigned)g.nz ? 32u : 0u);
+}
+
 template <class Real, class St, int SCHEME>
-__device__ __forceinline__ void ghost_images(const Geo& g, unsigned J, unsigned I, unsigned K, unsigned c,
-                                             const St& N, const CellVal<St>& v, Real* Tw, Real* tau,
-                                             const StepPar& par) {
+__device__ __forceinline__ void ghost_images(const Geo& g, unsigned imask, unsigned c, const St& N,
+                                             const CellVal<St>& v, Real* Tw, Real* tau, const StepPar& par) {
     constexpr bool prim = SCHEME != GKM_SCHEME_GKS;
     Real Tc = Real(0), tc = Real(0);
     if constexpr (prim) {
@@ -528,19 +533,15 @@ __device__ __forceinline__ void ghost_images(const Geo& g, unsigned J, unsigned 
         put(Tw, c, Tc);
         put(tau, c, tc);
     }
-    const unsigned n[3] = {(unsigned)g.ny, (unsigned)g.nx, (unsigned)g.nz};
-    const unsigned P[3] = {J, I, K};
-    const int st[3] = {(int)g.s0, (int)g.s1, 1};
-    bool any = false;
+    if (!imask) return;
+    const int n[3] = {g.ny * (int)g.s0, g.nx * (int)g.s1, g.nz};
     int off[3][3];  // per axis: own position, low image (+n), high image (-n); 0 = none
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         off[a][0] = 0;
-        off[a][1] = P[a] < 6u ? (int)n[a] * st[a] : 0;
-        off[a][2] = P[a] >= n[a] ? -(int)n[a] * st[a] : 0;
-        any = any || off[a][1] != 0 || off[a][2] != 0;
+        off[a][1] = (imask >> (2 * a)) & 1u ? n[a] : 0;
+        off[a][2] = (imask >> (2 * a + 1)) & 1u ? -n[a] : 0;
     }
-    if (!any) return;
     for (int x = 0; x < 3; x++) {
         if (x && !off[0][x]) continue;
         for (int y = 0; y < 3; y++) {
@@ -605,7 +606,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     const unsigned e = (gt >> 6) * 63u + lane;
     const bool valid = e < zl.count;
     const StepPar& par = pars[blockIdx.y];
-    unsigned c = 0;
+    unsigned c = 0, imask = 0;
     bool upd = false;  // lane 0's cell belongs to the previous wavefront; k = 2 is a ghost
     if (valid) {
         const unsigned line = fdiv(e, zl.fnI);
@@ -614,13 +615,18 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
         unsigned j, i;
         zline_column(zl, line, j, i);
         c = blockIdx.y * g.ncell + (j + 3) * g.s0 + (i + 3) * g.s1 + (kk + 2);
+        if constexpr (GHOST) imask = image_mask(g, j + 3, i + 3, kk + 2);
     }
     // the flags as a wave-uniform mask (scalar registers), taken before the
     // flux so that no per-lane index value stays live in a vector register
     // across it
     const unsigned long long upd_mask = __ballot(upd);
-    extern __shared__ unsigned char upd_lds_raw[];  // UpdSlots<St>::n * blockDim.x values
+    // UpdSlots<St>::n * blockDim.x values, then blockDim.x image masks
+    extern __shared__ unsigned char upd_lds_raw[];
     Real* upd_lds = reinterpret_cast<Real*>(upd_lds_raw);
+    unsigned* img_lds = reinterpret_cast<unsigned*>(upd_lds + UpdSlots<St>::n * blockDim.x);
+    // the image mask waits in LDS across the flux, like the update's operands
+    if constexpr (GHOST) img_lds[threadIdx.x] = imask;
     // Where the staging goes was measured (256^3): FP64, in a block of its own
     // ahead of the flux (fused kernel 2.05 -> 1.91 ms; the compiler issues it
     // first and waits for it before the flux's loads); FP32, in the flux's block
@@ -646,17 +652,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     if (!((upd_mask >> __lane_id()) & 1ull)) return;
     CellVal<St> v;
     update_finish(c, nx, Fz, Fm, upd_lds, par, v);
-    if constexpr (GHOST) {
-        // the cell's indices again, from the wavefront number and the lane
-        // (written apart from e's expression so that e is not kept live
-        // across the flux)
-        const unsigned e2 = (xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 63u + __lane_id();
-        const unsigned line = fdiv(e2, zl.fnI);
-        const unsigned kk = e2 - line * (zl.fnI.d);
-        unsigned j, i;
-        zline_column(zl, line, j, i);
-        ghost_images<Real, St, SCHEME>(g, j + 3, i + 3, kk + 2, c, nx, v, Tw_nx, tau_nx, par);
-    }
+    if constexpr (GHOST) ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -896,7 +892,7 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
     const int bs = h->flux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
-    const size_t lds = sizeof(Real) * UpdSlots<St>::n * (size_t)bs;
+    const size_t lds = (sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)) * (size_t)bs;
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     auto fused = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(bs), lds, h->stream, g, zl, st1, st2, st, nx, tau, wi, F, h->d_pars,
