@@ -351,6 +351,24 @@ def main():
     for _ in range(3):
         mixed.print_stats()
     print_stats_ms = 1e3 * (time.perf_counter() - t_ps) / 3
+    # the reference's loop body, PrintStats then the step (update.cpp:131-137):
+    # called in turn, and with each state's statistics beside the step that
+    # reads it (gkm_step_print_stats); 10 iterations each
+    mixed.sync()
+    t_l = time.perf_counter()
+    for _ in range(10):
+        mixed.print_stats()
+        mixed.evolve_eulerian(1, dt=dt)
+    mixed.sync()
+    loop_turn_ms = 1e3 * (time.perf_counter() - t_l) / 10
+    mixed.step_print_stats(2, dt)  # first use: second stream, pinned rows
+    mixed.sync()
+    t_l = time.perf_counter()
+    mixed.step_print_stats(10, dt)
+    mixed.sync()
+    loop_overlap_ms = 1e3 * (time.perf_counter() - t_l) / 10
+    print(f"[bench] PrintStats + step: {loop_turn_ms:.2f} ms in turn, {loop_overlap_ms:.2f} ms overlapped",
+          file=sys.stderr, flush=True)
     mixed.close()
     rate_m = cells * args.steps / sec_m
 
@@ -424,6 +442,8 @@ def main():
         "fp64": res_fp64,
         "mixed_over_fp64": (rate_m / res_fp64["value"]) if res_fp64 else None,
         "print_stats_ms": print_stats_ms,
+        "evolve_loop_ms_per_iteration": {"print_stats_then_step": loop_turn_ms,
+                                         "overlapped": loop_overlap_ms},
         "ek_rel_err": ek_err,
         "mt_rel_err": mt_err,
         "cpu_baseline": None,

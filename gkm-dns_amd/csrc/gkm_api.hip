@@ -99,7 +99,12 @@ void free_ctx(Ctx* h) {
     if (h->spl_grad) (void)hipFree(h->spl_grad);
     if (h->spl_inv) (void)hipFree(h->spl_inv);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
+    if (h->ps_host) (void)hipHostFree(h->ps_host);
     if (h->arena) (void)hipFree(h->arena);
+    if (h->ev_state) (void)hipEventDestroy(h->ev_state);
+    for (hipEvent_t e : h->ev_ps)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -720,35 +725,35 @@ int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
 
 int gkm_stats(gkm_t* h, gkm_stats_t* out) { return gkm_stats_box(h, 0, out); }
 
-int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
-    Ctx* h = ctx(hh);
-    int rc = check_box(h, box);
-    if (rc) return rc;
-    if (!out) return fail(GKM_E_ARG, "null argument");
+// the nine gradient planes of one box and the Thomas pivots of the three
+// line lengths, allocated on first use
+static int ensure_spline_buffers(Ctx* h) {
+    if (h->spl_grad) return GKM_OK;
     const gkm::Geo& g = h->geo;
-    if (!h->spl_grad) {
-        // Thomas pivots of the natural-spline slope system for each line length
-        std::vector<double> inv;
-        for (int m : {g.P0, g.P1, g.P2}) {
-            double prev = 0.0;
-            for (int i = 0; i < m; i++) {
-                const double b = (i == 0 || i == m - 1) ? 2.0 : 4.0;
-                prev = 1.0 / (i == 0 ? b : b - prev);
-                inv.push_back(prev);
-            }
+    // Thomas pivots of the natural-spline slope system for each line length
+    std::vector<double> inv;
+    for (int m : {g.P0, g.P1, g.P2}) {
+        double prev = 0.0;
+        for (int i = 0; i < m; i++) {
+            const double b = (i == 0 || i == m - 1) ? 2.0 : 4.0;
+            prev = 1.0 / (i == 0 ? b : b - prev);
+            inv.push_back(prev);
         }
-        hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
-        if (e != hipSuccess) {
-            h->spl_grad = nullptr;
-            return fail(GKM_E_NOMEM, "gradient planes (%zu bytes): %s", sizeof(double) * 9 * (size_t)h->ncell,
-                        hipGetErrorString(e));
-        }
-        HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
-        HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     }
-    gkm::ensure_ghosts(h);
-    double a[gkm::kPStat];
-    HIP_TRY(gkm::launch_print_stats(h, box, a));
+    hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
+    if (e != hipSuccess) {
+        h->spl_grad = nullptr;
+        return fail(GKM_E_NOMEM, "gradient planes (%zu bytes): %s", sizeof(double) * 9 * (size_t)h->ncell,
+                    hipGetErrorString(e));
+    }
+    HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
+    HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    return GKM_OK;
+}
+
+// the PrintStats quantities from the device sums a (kPStat) and the squared
+// deviations b (kStatB) about the p, T and 1/rho means
+static int fill_print_stats(const Ctx* h, const double* a, const double* b, gkm_print_stats_t* out) {
     const double N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
     gkm_print_stats_t s;
     std::memset(&s, 0, sizeof s);
@@ -770,16 +775,85 @@ int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
     s.p_mean = a[12] / N;
     s.T_mean = a[13] / N;
     s.v_mean = a[14] / N;
-    // second pass: squared deviations of (rho, p, T, 1/rho) about the means
-    const double m[4] = {0.0, s.p_mean, s.T_mean, s.v_mean};
-    double b[gkm::kStatB];
-    HIP_TRY(gkm::launch_stats_b(h, box, m, b));
     s.p_rms = std::sqrt(b[1] / N);
     s.T_rms = std::sqrt(b[2] / N);
     s.v_rms = std::sqrt(b[3] / N);
     *out = s;
     if (!(s.p_mean > 0) || !(s.v_mean > 0) || !std::isfinite(s.tke))
         return fail(GKM_E_NONPHYSICAL, "non-physical state: <p>=%g <1/rho>=%g", s.p_mean, s.v_mean);
+    return GKM_OK;
+}
+
+int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
+    Ctx* h = ctx(hh);
+    int rc = check_box(h, box);
+    if (rc) return rc;
+    if (!out) return fail(GKM_E_ARG, "null argument");
+    rc = ensure_spline_buffers(h);
+    if (rc) return rc;
+    gkm::ensure_ghosts(h);
+    double a[gkm::kPStat + gkm::kStatB];
+    HIP_TRY(gkm::launch_print_stats_async(h, box, h->stream, a));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return fill_print_stats(h, a, a + gkm::kPStat, out);
+}
+
+int gkm_step_print_stats(gkm_t* hh, double dt, int nsteps, gkm_print_stats_t* out) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (h->nbox != 1) return fail(GKM_E_ARG, "handle holds %d boxes: gkm_step_print_stats takes one box", h->nbox);
+    if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
+    if (nsteps > 0 && !out) return fail(GKM_E_ARG, "null argument");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(GKM_E_ARG, "dt must be positive and finite");
+    int rc = ensure_spline_buffers(h);
+    if (rc) return rc;
+    if (h->halo_mode == GKM_HALO_PACK) {
+        // every step starts with the exchange, which rewrites the ghosts the
+        // statistics read: no overlap, the loop as written
+        for (int s = 0; s < nsteps; s++) {
+            rc = gkm_print_stats_box(hh, 0, out + s);
+            if (rc) return rc;
+            rc = gkm_step_batch(hh, &dt, 1);
+            if (rc) return rc;
+        }
+        return GKM_OK;
+    }
+    if (!h->stream2) {
+        HIP_TRY(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming));
+        for (hipEvent_t& e : h->ev_ps) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    constexpr int row = gkm::kPStat + gkm::kStatB;
+    if (h->ps_rows < nsteps) {
+        if (h->ps_host) HIP_TRY(hipHostFree(h->ps_host));
+        h->ps_host = nullptr;
+        h->ps_rows = 0;
+        HIP_TRY(hipHostMalloc((void**)&h->ps_host, sizeof(double) * row * (size_t)nsteps, hipHostMallocDefault));
+        h->ps_rows = nsteps;
+    }
+    rc = upload_pars(h, &dt);
+    if (rc) return rc;
+    gkm::prepare_step(h);  // ghosts (and primitives) of the first state, on the step stream
+    for (int s = 0; s < nsteps; s++) {
+        // statistics of state s on the second stream, once the step that wrote it is done
+        HIP_TRY(hipEventRecord(h->ev_state, h->stream));
+        HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_state, 0));
+        HIP_TRY(gkm::launch_print_stats_async(h, 0, h->stream2, h->ps_host + (size_t)row * s));
+        HIP_TRY(hipEventRecord(h->ev_ps[s & 1], h->stream2));
+        // step s reads state s (as the statistics do) and writes the other buffer,
+        // which the statistics of state s-1 read: it waits for those only
+        if (s > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(s - 1) & 1], 0));
+        rc = gkm_step_batch(hh, &dt, 1);
+        if (rc) return rc;
+    }
+    // the handle's later work (which may write the buffer the last statistics read) follows them
+    if (nsteps > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(nsteps - 1) & 1], 0));
+    HIP_TRY(hipStreamSynchronize(h->stream2));
+    for (int s = 0; s < nsteps; s++) {
+        const double* a = h->ps_host + (size_t)row * s;
+        rc = fill_print_stats(h, a, a + gkm::kPStat, out + s);
+        if (rc) return rc;
+    }
     return GKM_OK;
 }
 

@@ -74,6 +74,12 @@ struct Ctx {
     // of the three line lengths (P0, P1, P2 values), allocated on first use
     double* spl_grad = nullptr;
     double* spl_inv = nullptr;
+    // gkm_step_print_stats: a second stream (default priority) for the
+    // statistics, its events and the pinned host rows of the sums
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_state = nullptr, ev_ps[2] = {nullptr, nullptr};
+    double* ps_host = nullptr;
+    int ps_rows = 0;
     // halo mode and the y-face exchange buffers of GKM_HALO_PACK
     int halo_mode = GKM_HALO_GATHER;
     void* halo_send = nullptr;
@@ -112,6 +118,6 @@ hipError_t launch_stats_a(Ctx* h, int box, double* host_out /*kStatA*/);
 hipError_t launch_stats_b(Ctx* h, int box, const double means[4], double* host_out /*kStatB*/);
 constexpr int kPStat = 15;  // PrintStats sums
 // spline slopes into h->spl_grad (allocated by the caller), then the sums
-hipError_t launch_print_stats(Ctx* h, int box, double* host_out /*kPStat*/);
+hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host_out /*kPStat + kStatB*/);
 
 }  // namespace gkm

@@ -130,10 +130,24 @@ __global__ void __launch_bounds__(kRedThreads) k_stats_a(Geo g, St st, StatPar p
     block_reduce_store<kStatA, (1u << 27)>(acc, part);
 }
 
-// pass B: squared deviations of rho, p, T, 1/rho about the given means
+// pass B: squared deviations of rho, p, T, 1/rho about the given means, or,
+// with msrc, about msrc[i] / N for slots mslot[i] of an earlier pass's sums
+// on the device (the same division the host makes of them)
+struct MeanSrc {
+    const double* sums;
+    int slot[4];  // < 0: mean 0
+    double N;
+};
 template <class St>
 __global__ void __launch_bounds__(kRedThreads) k_stats_b(Geo g, St st, StatPar p, double m0, double m1, double m2,
-                                                         double m3, double* part) {
+                                                         double m3, MeanSrc msrc, double* part) {
+    if (msrc.sums) {
+        auto mean = [&](int i) { return msrc.slot[i] < 0 ? 0.0 : msrc.sums[msrc.slot[i]] / msrc.N; };
+        m0 = mean(0);
+        m1 = mean(1);
+        m2 = mean(2);
+        m3 = mean(3);
+    }
     double acc[kStatB] = {0.0, 0.0, 0.0, 0.0};
     const long n = (long)g.nx * g.ny * g.nz;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -216,7 +230,9 @@ hipError_t launch_stats_a(Ctx* h, int box, double* host_out) {
     return hipStreamSynchronize(h->stream);
 }
 
-hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) {
+// pass B into out (device), on stream s
+static void launch_stats_b_dev(Ctx* h, int box, const double m[4], const MeanSrc& msrc, hipStream_t s,
+                               double* out) {
     const Geo& g = h->geo;
     const StatPar p = stat_par(h, box);
     const int nb = h->red_blocks;
@@ -224,19 +240,22 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
-        hipLaunchKernelGGL(k_stats_b<StateD>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
-                           m[3], h->red_part);
+        hipLaunchKernelGGL(k_stats_b<StateD>, dim3(nb), dim3(kRedThreads), 0, s, g, st, p, m[0], m[1], m[2], m[3],
+                           msrc, h->red_part);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q] + off;
             st.lo[q] = h->Wlo[q] + off;
         }
-        hipLaunchKernelGGL(k_stats_b<StateM>, dim3(nb), dim3(kRedThreads), 0, h->stream, g, st, p, m[0], m[1], m[2],
-                           m[3], h->red_part);
+        hipLaunchKernelGGL(k_stats_b<StateM>, dim3(nb), dim3(kRedThreads), 0, s, g, st, p, m[0], m[1], m[2], m[3],
+                           msrc, h->red_part);
     }
-    hipLaunchKernelGGL((k_reduce_final<kStatB, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, nb, h->red_part,
-                       h->red_out);
+    hipLaunchKernelGGL((k_reduce_final<kStatB, 0u>), dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part, out);
+}
+
+hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) {
+    launch_stats_b_dev(h, box, m, MeanSrc{}, h->stream, h->red_out);
     hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kStatB, hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(h->stream);
@@ -703,7 +722,7 @@ __global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const doub
 // they fit, m <= kSplMaxM, else in chunks), then the sums; returns the number
 // of partial rows.
 template <class St>
-static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
+static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStream_t s) {
     const Geo& g = h->geo;
     const double r3h = 3.0 / h->grid.dx;
     const double* inv[3] = {h->spl_inv, h->spl_inv + g.P0, h->spl_inv + g.P0 + g.P1};
@@ -714,40 +733,51 @@ static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p) {
         constexpr int A = decltype(Ax)::value;
         if (m[A] <= kSplMaxM)
             hipLaunchKernelGGL((k_spl_lines<St, A>), dim3((nl[A] + kSplL - 1) / kSplL), dim3(kSplThreads),
-                               tile_bytes(m[A]), h->stream, g, st, inv[A], r3h, h->spl_grad);
+                               tile_bytes(m[A]), s, g, st, inv[A], r3h, h->spl_grad);
         else
             hipLaunchKernelGGL((k_spl_chunked<St, A>), dim3((nl[A] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0,
-                               h->stream, g, st, inv[A], r3h, h->spl_grad);
+                               s, g, st, inv[A], r3h, h->spl_grad);
     };
     axis(std::integral_constant<int, 2>());
     axis(std::integral_constant<int, 1>());
     axis(std::integral_constant<int, 0>());
-    hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, h->stream, g, st, h->spl_grad, p,
+    hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, s, g, st, h->spl_grad, p,
                        make_fastdiv((unsigned)g.nz), make_fastdiv((unsigned)g.nx), h->red_part);
     return h->red_blocks;
 }
 
-hipError_t launch_print_stats(Ctx* h, int box, double* host_out) {
+// The PrintStats sums (kPStat) and, about their means, the squared
+// deviations of pass B (kStatB), both reduced on the device and copied to
+// host_out (kPStat + kStatB doubles) by one copy, all on stream s: no host
+// round trip between the two passes.  Returns without waiting.
+hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host_out) {
     const StatPar p = stat_par(h, box);
     const size_t off = (size_t)box * h->ncell;
     int nb;
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
-        nb = launch_print_stats_st(h, st, p);
+        nb = launch_print_stats_st(h, st, p, s);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q] + off;
             st.lo[q] = h->Wlo[q] + off;
         }
-        nb = launch_print_stats_st(h, st, p);
+        nb = launch_print_stats_st(h, st, p, s);
     }
-    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, h->stream, nb, h->red_part,
-                       h->red_out);
-    hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kPStat, hipMemcpyDeviceToHost, h->stream);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(h->stream);
+    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part, h->red_out);
+    // pass B about p, T and 1/rho means (slots 12..14; rho's slot is not reported)
+    MeanSrc ms;
+    ms.sums = h->red_out;
+    ms.slot[0] = -1;
+    ms.slot[1] = 12;
+    ms.slot[2] = 13;
+    ms.slot[3] = 14;
+    ms.N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
+    const double m0[4] = {0.0, 0.0, 0.0, 0.0};
+    launch_stats_b_dev(h, box, m0, ms, s, h->red_out + kPStat);
+    return hipMemcpyAsync(host_out, h->red_out, sizeof(double) * (kPStat + kStatB), hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace gkm

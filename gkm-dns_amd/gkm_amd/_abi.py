@@ -34,7 +34,7 @@ EXPORTS = (
     "gkm_weno_points", "gkm_last_error", "gkm_version",
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
-    "gkm_print_stats", "gkm_print_stats_box", "gkm_format_print_stats", "gkm_gks_points",
+    "gkm_print_stats", "gkm_print_stats_box", "gkm_step_print_stats", "gkm_format_print_stats", "gkm_gks_points",
     "gkm_last_sweep_timing", "gkm_get_grid", "gkm_get_params_box", "gkm_set_params_box",
     "gkm_set_dt_batch", "gkm_group_create", "gkm_group_step", "gkm_group_sync", "gkm_group_destroy",
     # include/gkm_ic.h
@@ -160,6 +160,7 @@ def load() -> C.CDLL:
     L.gkm_set_halo_mode.argtypes = [vp, C.c_int]
     L.gkm_print_stats.argtypes = [vp, C.POINTER(GkmPrintStats)]
     L.gkm_print_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmPrintStats)]
+    L.gkm_step_print_stats.argtypes = [vp, C.c_double, C.c_int, C.POINTER(GkmPrintStats)]
     L.gkm_format_print_stats.argtypes = [C.POINTER(GkmPrintStats)] + [C.c_double] * 6 + [C.c_char_p, C.c_int]
     FP = C.POINTER(C.c_float)
     IP = C.POINTER(C.c_int)

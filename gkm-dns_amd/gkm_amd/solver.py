@@ -212,6 +212,33 @@ class Solver:
         _abi.check(self._L.gkm_stats(self._h, C.byref(s)))
         return s.as_dict()
 
+    def _line(self, s: dict, t: float, Mt=None, p0=None, T0=None, rho0=None) -> str:
+        norm = getattr(self, "norm", None) or {}
+        Mt = Mt if Mt is not None else norm.get("Mt", 1.0)
+        p0 = p0 if p0 is not None else norm.get("p0", s["p_mean"])
+        rho0 = rho0 if rho0 is not None else norm.get("rho0", 1.0)
+        T0 = T0 if T0 is not None else self.gas.T0
+        return _abi.format_print_stats(s, t / self.info.t0, self.gas.gamma, p0, T0, Mt, rho0)
+
+    def step_print_stats(self, nsteps: int, dt: float | None = None) -> list:
+        """nsteps of the Evolve loop body, PrintStats then the step, with the
+        statistics of each state overlapping the step that reads it
+        (gkm_step_print_stats).  Returns the statistics dicts (with their
+        "line") of the states before each step."""
+        dt = self.info.dt if dt is None else dt
+        if not dt > 0:
+            raise ValueError("dt is not set: call timestep() or pass dt")
+        out = (_abi.GkmPrintStats * max(1, int(nsteps)))()
+        _abi.check(self._L.gkm_step_print_stats(self._h, float(dt), int(nsteps), out))
+        res = []
+        for k in range(int(nsteps)):
+            s = out[k].as_dict()
+            s["line"] = self._line(s, self.info.t)
+            res.append(s)
+            self.info.t += dt
+            self.info.steps += 1
+        return res
+
     def print_stats(self, Mt: float | None = None, p0: float | None = None, T0: float | None = None,
                     rho0: float | None = None, file=None) -> dict:
         """PrintStats (update.cpp:638-784): natural-spline velocity gradients
@@ -222,12 +249,7 @@ class Solver:
         st = _abi.GkmPrintStats()
         _abi.check(self._L.gkm_print_stats(self._h, C.byref(st)))
         s = st.as_dict()
-        norm = getattr(self, "norm", None) or {}
-        Mt = Mt if Mt is not None else norm.get("Mt", 1.0)
-        p0 = p0 if p0 is not None else norm.get("p0", s["p_mean"])
-        rho0 = rho0 if rho0 is not None else norm.get("rho0", 1.0)
-        T0 = T0 if T0 is not None else self.gas.T0
-        s["line"] = _abi.format_print_stats(s, self.info.t / self.info.t0, self.gas.gamma, p0, T0, Mt, rho0)
+        s["line"] = self._line(s, self.info.t, Mt, p0, T0, rho0)
         if file is not None:
             print(s["line"], file=file)
         return s
@@ -239,6 +261,19 @@ class Solver:
         self.periodic_bc()
         if self.info.dt <= 0 or recompute_dt:
             self.timestep(cfl)
+        if stats_every == 1 and not recompute_dt and callback is None:
+            # statistics before every step at a fixed dt: the overlapped loop body
+            n, t = 0, self.info.t
+            while t <= t_end:
+                n += 1
+                t += self.info.dt
+            t = self.info.t
+            for s in self.step_print_stats(n):
+                self.info.history.append((t, s))
+                t += self.info.dt
+            s = self.print_stats()
+            self.info.history.append((self.info.t, s))
+            return self.info.history
         while self.info.t <= t_end:
             if self.info.steps % stats_every == 0:
                 s = self.print_stats()

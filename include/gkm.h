@@ -229,6 +229,17 @@ int gkm_stats(gkm_t* h, gkm_stats_t* out);
  * FP64 gradient planes of one box (freed by gkm_destroy). */
 int gkm_print_stats(gkm_t* h, gkm_print_stats_t* out);
 int gkm_print_stats_box(gkm_t* h, int box, gkm_print_stats_t* out);
+/* The body of the reference's Evolve loop (weno/src/update.cpp:131-146:
+ * PrintStats of the current state, then the step, t += dt), nsteps times with
+ * a fixed dt: out[s] receives PrintStats of the state before step s (nsteps
+ * entries).  One-box handles.  The statistics of state s run on a second
+ * stream of the handle while step s runs (both read state s; step s writes
+ * the other state buffer), so each pair costs about the longer of the two
+ * rather than their sum.  The results equal gkm_print_stats then gkm_step
+ * called in turn; with GKM_HALO_PACK the two are called in turn.  Returns once
+ * the last statistics are on the host (the last step may still run, as after
+ * gkm_step). */
+int gkm_step_print_stats(gkm_t* h, double dt, int nsteps, gkm_print_stats_t* out);
 /* PrintStats' stdout line (update.cpp:778-781), "%.8e" columns:
  *   t/t0  TKE  Diss  SKEWu  SKEWv  SKEWw  KURTu  KURTv  KURTw
  *   Prms/(gamma p0 Mt^2)  Trms/((gamma-1) T0 Mt^2)  Vrms/(Mt^2/den0)

@@ -139,3 +139,44 @@ def test_print_stats_batch_equals_single(gk, orc):
             s.set_state(W)
             d = s.print_stats()
             assert {k: d[k] for k in got[i]} == got[i]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("scheme", ["weno_bgk", "gks"])
+def test_step_print_stats_matches_calls_in_turn(gk, prec, scheme):
+    """gkm_step_print_stats (statistics of each state on a second stream beside
+    the step that reads it) gives, bitwise, the statistics and states of
+    gkm_print_stats and gkm_step called in turn: eager and graph steps, and
+    the slab-exchange mode, which runs them in turn."""
+    kw = dict(precision=prec, visc_law="power")
+    if scheme == "gks":
+        kw.update(flux_scheme="gks", Pr=1.0)
+    else:
+        kw.update(recon="js")
+    ic = gk.dhit(14, ny=12, nz=10, Mt=0.3, Re=40, seed=123)
+    for graph, halo in [(False, "gather"), (True, "gather"), (False, "pack")]:
+        print("step_print_stats", prec, scheme, graph, halo, flush=True)
+        a = gk.Solver.from_ic(ic, **kw)
+        b = gk.Solver.from_ic(ic, **kw)
+        a.set_graph(graph)
+        b.set_graph(graph)
+        a.set_halo_mode(halo)
+        b.set_halo_mode(halo)
+        dt = a.timestep(0.4)
+        got = a.step_print_stats(5, dt)
+        want = []
+        for _ in range(5):
+            want.append(b.print_stats())
+            b.evolve_eulerian(1, dt=dt)
+        for g_, w_ in zip(got, want):
+            assert g_ == w_
+        for x, y in zip(a.get_state(), b.get_state()):
+            assert np.array_equal(x, y)
+        # and again from where they are (second call reuses the stream and buffers)
+        got = a.step_print_stats(2, dt)
+        for g_ in got:
+            w_ = b.print_stats()
+            b.evolve_eulerian(1, dt=dt)
+            assert g_ == w_
+        a.close()
+        b.close()
