@@ -24,7 +24,8 @@ struct Ctx {
     int nbox = 1;
     int device;
     hipStream_t stream;
-    int flux_block = 256;
+    int flux_block = 256;  // X and Y flux kernels (gkm_set_tuning)
+    int zflux_block = 128;  // the fused Z kernel: 128 (mixed) or 64 (FP64), set at creation
     int recon_chunk = 8;
     // per-box step parameters: device array read by the kernels, pinned staging
     StepPar* d_pars = nullptr;

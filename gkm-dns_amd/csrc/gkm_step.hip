@@ -890,7 +890,7 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     zl.fW = make_fastdiv(W);
     zl.fWl = make_fastdiv(wl ? wl : W);
     const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
-    const int bs = h->flux_block;
+    const int bs = h->zflux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
     const size_t lds = (sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)) * (size_t)bs;
     if (ev) (void)hipEventRecord(ev[0], h->stream);

@@ -23,7 +23,8 @@ from .solver import Batch
 
 FLUX_BLOCKS = (64, 128, 256)
 RECON_CHUNKS = (4, 8, 16)
-DEFAULT = (256, 8)
+# the library's defaults (gkm_create) per precision
+DEFAULT = {"mixed": (256, 8), "fp64": (64, 8)}
 
 
 def cache_path() -> Path:
@@ -72,7 +73,7 @@ def tune_size(n: int, precision: str = "mixed", recon: str = "js", nbox: int = 4
     b.close()
     best = min(times, key=times.get)
     cache[key] = {"best": list(best), "ms_per_step": {f"{k[0]}x{k[1]}": 1e3 * v for k, v in times.items()},
-                  "default_ms": 1e3 * times[DEFAULT]}
+                  "default_ms": 1e3 * times[DEFAULT[precision]]}
     _store(path, cache)
     return best
 

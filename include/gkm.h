@@ -191,9 +191,10 @@ int gkm_group_step(gkm_group_t* g, int nsteps);
 int gkm_group_sync(gkm_group_t* g);
 int gkm_group_destroy(gkm_group_t* g);
 
-/* Launch shapes: threads per block of the flux kernels (64, 128 or 256) and
- * interfaces per thread of the strided-sweep WENO kernels (4, 8 or 16).
- * Chosen per box size by the autotuner in gkm_amd/tune.py. */
+/* Launch shapes: threads per block of the X and Y flux kernels (64, 128 or
+ * 256) and interfaces per thread of the strided-sweep WENO kernels (4, 8 or
+ * 16).  Chosen per box size by the autotuner in gkm_amd/tune.py.  (The fused
+ * Z sweep's block size is fixed per precision.) */
 int gkm_set_tuning(gkm_t* h, int flux_block, int recon_chunk);
 
 /* Copy the padded host state in / out (bit-exact round trip for FP64; the
