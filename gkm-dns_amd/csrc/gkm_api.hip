@@ -804,6 +804,27 @@ int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
     return fill_print_stats(h, a, a + gkm::kPStat, out);
 }
 
+// the loop of gkm_step_print_stats: statistics of state s on stream2 beside step s
+static int step_stats_loop(gkm_t* hh, Ctx* h, double dt, int nsteps) {
+    constexpr int row = gkm::kPStat + gkm::kStatB;
+    gkm::prepare_step(h);  // ghosts (and primitives) of the first state, on the step stream
+    for (int s = 0; s < nsteps; s++) {
+        // statistics of state s on the second stream, once the step that wrote it is done
+        HIP_TRY(hipEventRecord(h->ev_state, h->stream));
+        HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_state, 0));
+        HIP_TRY(gkm::launch_print_stats_async(h, 0, h->stream2, h->ps_host + (size_t)row * s));
+        HIP_TRY(hipEventRecord(h->ev_ps[s & 1], h->stream2));
+        // step s reads state s (as the statistics do) and writes the other buffer,
+        // which the statistics of state s-1 read: it waits for those only
+        if (s > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(s - 1) & 1], 0));
+        const int rc = gkm_step_batch(hh, &dt, 1);
+        if (rc) return rc;
+    }
+    // the handle's later work (which may write the buffer the last statistics read) follows them
+    if (nsteps > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(nsteps - 1) & 1], 0));
+    return GKM_OK;
+}
+
 int gkm_step_print_stats(gkm_t* hh, double dt, int nsteps, gkm_print_stats_t* out) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
@@ -824,11 +845,11 @@ int gkm_step_print_stats(gkm_t* hh, double dt, int nsteps, gkm_print_stats_t* ou
         }
         return GKM_OK;
     }
-    if (!h->stream2) {
-        HIP_TRY(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming));
-        for (hipEvent_t& e : h->ev_ps) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
+    // created on first use (each one that a failed earlier attempt left missing)
+    if (!h->stream2) HIP_TRY(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    if (!h->ev_state) HIP_TRY(hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming));
+    for (hipEvent_t& e : h->ev_ps)
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     constexpr int row = gkm::kPStat + gkm::kStatB;
     if (h->ps_rows < nsteps) {
         if (h->ps_host) HIP_TRY(hipHostFree(h->ps_host));
@@ -839,22 +860,11 @@ int gkm_step_print_stats(gkm_t* hh, double dt, int nsteps, gkm_print_stats_t* ou
     }
     rc = upload_pars(h, &dt);
     if (rc) return rc;
-    gkm::prepare_step(h);  // ghosts (and primitives) of the first state, on the step stream
-    for (int s = 0; s < nsteps; s++) {
-        // statistics of state s on the second stream, once the step that wrote it is done
-        HIP_TRY(hipEventRecord(h->ev_state, h->stream));
-        HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_state, 0));
-        HIP_TRY(gkm::launch_print_stats_async(h, 0, h->stream2, h->ps_host + (size_t)row * s));
-        HIP_TRY(hipEventRecord(h->ev_ps[s & 1], h->stream2));
-        // step s reads state s (as the statistics do) and writes the other buffer,
-        // which the statistics of state s-1 read: it waits for those only
-        if (s > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(s - 1) & 1], 0));
-        rc = gkm_step_batch(hh, &dt, 1);
-        if (rc) return rc;
-    }
-    // the handle's later work (which may write the buffer the last statistics read) follows them
-    if (nsteps > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_ps[(nsteps - 1) & 1], 0));
-    HIP_TRY(hipStreamSynchronize(h->stream2));
+    rc = step_stats_loop(hh, h, dt, nsteps);
+    // nothing may still write the pinned rows once the call has returned
+    const hipError_t e2 = hipStreamSynchronize(h->stream2);
+    if (rc) return rc;
+    if (e2 != hipSuccess) return fail(GKM_E_HIP, "statistics stream: %s", hipGetErrorString(e2));
     for (int s = 0; s < nsteps; s++) {
         const double* a = h->ps_host + (size_t)row * s;
         rc = fill_print_stats(h, a, a + gkm::kPStat, out + s);
