@@ -130,24 +130,10 @@ __global__ void __launch_bounds__(kRedThreads) k_stats_a(Geo g, St st, StatPar p
     block_reduce_store<kStatA, (1u << 27)>(acc, part);
 }
 
-// pass B: squared deviations of rho, p, T, 1/rho about the given means, or,
-// with msrc, about msrc[i] / N for slots mslot[i] of an earlier pass's sums
-// on the device (the same division the host makes of them)
-struct MeanSrc {
-    const double* sums;
-    int slot[4];  // < 0: mean 0
-    double N;
-};
+// pass B: squared deviations of rho, p, T, 1/rho about the given means
 template <class St>
 __global__ void __launch_bounds__(kRedThreads) k_stats_b(Geo g, St st, StatPar p, double m0, double m1, double m2,
-                                                         double m3, MeanSrc msrc, double* part) {
-    if (msrc.sums) {
-        auto mean = [&](int i) { return msrc.slot[i] < 0 ? 0.0 : msrc.sums[msrc.slot[i]] / msrc.N; };
-        m0 = mean(0);
-        m1 = mean(1);
-        m2 = mean(2);
-        m3 = mean(3);
-    }
+                                                         double m3, double* part) {
     double acc[kStatB] = {0.0, 0.0, 0.0, 0.0};
     const long n = (long)g.nx * g.ny * g.nz;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -170,27 +156,41 @@ __global__ void __launch_bounds__(kRedThreads) k_stats_b(Geo g, St st, StatPar p
     block_reduce_store<kStatB, 0u>(acc, part);
 }
 
-// final pass: one block reduces nb partial rows of NQ slots in block order
+// final pass: one block (kRedThreads) reduces nb partial rows of NQ slots in
+// a fixed order: thread t takes rows t, t + kRedThreads, ... in turn, all
+// slots in one sweep over the rows; each slot then goes through the wavefront
+// sums and the wavefronts' values in order.  Thread q < NQ writes res[q].
+template <int NQ, unsigned MAXMASK>
+__device__ __forceinline__ void reduce_rows(int nb, const double* __restrict__ part, double* res) {
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const double x = part[(long)b * NQ + q];
+            v[q] = ((MAXMASK >> q) & 1u) ? fmax(v[q], x) : v[q] + x;
+        }
+    }
+    __shared__ double lds[NQ][kRedThreads / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const double w = ((MAXMASK >> q) & 1u) ? wave_max(v[q]) : wave_sum(v[q]);
+        if (lane == 0) lds[q][wv] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        double t = lds[q][0];
+        for (int w = 1; w < kRedThreads / 64; w++) t = ((MAXMASK >> q) & 1u) ? fmax(t, lds[q][w]) : t + lds[q][w];
+        res[q] = t;
+    }
+}
+
 template <int NQ, unsigned MAXMASK>
 __global__ void __launch_bounds__(kRedThreads) k_reduce_final(int nb, const double* part, double* out) {
-    for (int q = 0; q < NQ; q++) {
-        const bool ismax = (MAXMASK >> q) & 1u;
-        double v = ismax ? 0.0 : 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const double x = part[(long)b * NQ + q];
-            v = ismax ? fmax(v, x) : v + x;
-        }
-        v = ismax ? wave_max(v) : wave_sum(v);
-        __shared__ double lds[kRedThreads / 64];
-        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = lds[0];
-            for (int w = 1; w < kRedThreads / 64; w++) s = ismax ? fmax(s, lds[w]) : s + lds[w];
-            out[q] = s;
-        }
-        __syncthreads();
-    }
+    reduce_rows<NQ, MAXMASK>(nb, part, out);
 }
 
 static StatPar stat_par(const Ctx* h, int box) {
@@ -231,8 +231,7 @@ hipError_t launch_stats_a(Ctx* h, int box, double* host_out) {
 }
 
 // pass B into out (device), on stream s
-static void launch_stats_b_dev(Ctx* h, int box, const double m[4], const MeanSrc& msrc, hipStream_t s,
-                               double* out) {
+static void launch_stats_b_dev(Ctx* h, int box, const double m[4], hipStream_t s, double* out) {
     const Geo& g = h->geo;
     const StatPar p = stat_par(h, box);
     const int nb = h->red_blocks;
@@ -241,7 +240,7 @@ static void launch_stats_b_dev(Ctx* h, int box, const double m[4], const MeanSrc
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
         hipLaunchKernelGGL(k_stats_b<StateD>, dim3(nb), dim3(kRedThreads), 0, s, g, st, p, m[0], m[1], m[2], m[3],
-                           msrc, h->red_part);
+                           h->red_part);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
@@ -249,13 +248,13 @@ static void launch_stats_b_dev(Ctx* h, int box, const double m[4], const MeanSrc
             st.lo[q] = h->Wlo[q] + off;
         }
         hipLaunchKernelGGL(k_stats_b<StateM>, dim3(nb), dim3(kRedThreads), 0, s, g, st, p, m[0], m[1], m[2], m[3],
-                           msrc, h->red_part);
+                           h->red_part);
     }
     hipLaunchKernelGGL((k_reduce_final<kStatB, 0u>), dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part, out);
 }
 
 hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) {
-    launch_stats_b_dev(h, box, m, MeanSrc{}, h->stream, h->red_out);
+    launch_stats_b_dev(h, box, m, h->stream, h->red_out);
     hipError_t e = hipMemcpyAsync(host_out, h->red_out, sizeof(double) * kStatB, hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(h->stream);
@@ -287,19 +286,34 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
 // ---------------------------------------------------------------------------
 constexpr int kSpLines = 64, kSpThreads = 192, kSpChunk = 16;
 
+// density, |u|^2, pressure and temperature of cell c
+template <class St>
+__device__ __forceinline__ void cell_pt(const St& st, unsigned c, const StatPar& p, double& rho, double& q2,
+                                        double& P, double& T) {
+    rho = st.get(0, c);
+    double u[3];
+    vel(st, c, u);
+    q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
+    T = P / (p.R * rho);
+}
+
 // PrintStats sums over the interior (update.cpp:658-707):
 //  0 rho|u|^2  1 mu du_i/dx_j du_i/dx_j  2 S_ij S_ij  3..5 (du/dx)^2,3,4
 //  6..8 (dv/dy)^n  9..11 (dw/dz)^n  12 p  13 T  14 1/rho
+// and, for the rms of p, T and 1/rho (update.cpp:743-770 takes them about the
+// means in a second sweep), the sums of x - K (15..17) and (x - K)^2 (18..20)
+// about a shift K, the values of one fixed cell: sum (x - mean)^2 = S2 - S1^2/N
+// in exact arithmetic, and with K inside the range of the data the rounding
+// stays at that of the two-sweep form, so one sweep over the state does.
 // dX[m] = d(u_m)/dx, dY[m] = d(u_m)/dy, dZ[m] = d(u_m)/dz
+constexpr int kPSum = kPStat + 6;
 template <class St>
-__device__ __forceinline__ void pstat_add(const St& st, unsigned c, const StatPar& p, const double dX[3],
-                                          const double dY[3], const double dZ[3], double (&acc)[kPStat]) {
-    const double rho = st.get(0, c);
-    double u[3];
-    vel(st, c, u);
-    const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double P = (rho * (p.gam - 1)) * (st.get(4, c) / rho - 0.5 * q2);
-    const double T = P / (p.R * rho);
+__device__ __forceinline__ void pstat_add(const St& st, unsigned c, const StatPar& p, const double K[3],
+                                          const double dX[3], const double dY[3], const double dZ[3],
+                                          double (&acc)[kPSum]) {
+    double rho, q2, P, T;
+    cell_pt(st, c, p, rho, q2, P, T);
     const double dudx = dX[0], dudy = dY[0], dudz = dZ[0];
     const double dvdx = dX[1], dvdy = dY[1], dvdz = dZ[1];
     const double dwdx = dX[2], dwdy = dY[2], dwdz = dZ[2];
@@ -312,7 +326,11 @@ __device__ __forceinline__ void pstat_add(const St& st, unsigned c, const StatPa
     double x2 = dudx * dudx; acc[3] += x2; acc[4] += x2 * dudx; acc[5] += x2 * x2;
     x2 = dvdy * dvdy; acc[6] += x2; acc[7] += x2 * dvdy; acc[8] += x2 * x2;
     x2 = dwdz * dwdz; acc[9] += x2; acc[10] += x2 * dwdz; acc[11] += x2 * x2;
-    acc[12] += P; acc[13] += T; acc[14] += 1.0 / rho;
+    const double iv = 1.0 / rho;
+    acc[12] += P; acc[13] += T; acc[14] += iv;
+    const double dp = P - K[0], dT = T - K[1], dv = iv - K[2];
+    acc[15] += dp; acc[16] += dT; acc[17] += dv;
+    acc[18] += dp * dp; acc[19] += dT * dT; acc[20] += dv * dv;
 }
 
 // component q of the velocity at c
@@ -698,9 +716,15 @@ __global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_e
 template <class St>
 __global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const double* __restrict__ grad, StatPar p,
                                                         FastDiv fnz, FastDiv fnx, double* part) {
-    double acc[kPStat];
+    double acc[kPSum];
 #pragma unroll
-    for (int q = 0; q < kPStat; q++) acc[q] = 0.0;
+    for (int q = 0; q < kPSum; q++) acc[q] = 0.0;
+    double K[3];  // the deviation sums' shift: p, T, 1/rho of the first interior cell
+    {
+        double rho, q2;
+        cell_pt(st, 3 * g.s0 + 3 * g.s1 + 3, p, rho, q2, K[0], K[1]);
+        K[2] = 1.0 / rho;
+    }
     const unsigned n = (unsigned)g.nx * g.ny * g.nz;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const unsigned r = fdiv(t, fnz), k = t - r * fnz.d;
@@ -713,9 +737,26 @@ __global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const doub
             dX[q] = ld(grad + (size_t)(3 + q) * g.ncell, c);
             dZ[q] = ld(grad + (size_t)(6 + q) * g.ncell, c);
         }
-        pstat_add(st, c, p, dX, dY, dZ, acc);
+        pstat_add(st, c, p, K, dX, dY, dZ, acc);
     }
-    block_reduce_store<kPStat, 0u>(acc, part);
+    block_reduce_store<kPSum, 0u>(acc, part);
+}
+
+// the PrintStats row: the kPStat sums, then the kStatB slots of pass B (rho:
+// 0, not reported; p, T, 1/rho: the sums of squared deviations about the
+// means, S2 - S1 (S1 / N) from the shifted sums)
+__global__ void __launch_bounds__(kRedThreads) k_pstats_final(int nb, const double* part, double N, double* out) {
+    __shared__ double r[kPSum];
+    reduce_rows<kPSum, 0u>(nb, part, r);
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < kPStat) {
+        out[t] = r[t];
+    } else if (t < kPStat + kStatB) {
+        const int i = t - kPStat;  // 0 rho, 1 p, 2 T, 3 1/rho
+        const double s1 = i ? r[kPStat - 1 + i] : 0.0, s2 = i ? r[kPStat + 2 + i] : 0.0;
+        out[t] = fmax(0.0, s2 - s1 * (s1 / N));
+    }
 }
 
 // The slopes of the three axes into their planes (whole lines in LDS where
@@ -746,10 +787,10 @@ static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStre
     return h->red_blocks;
 }
 
-// The PrintStats sums (kPStat) and, about their means, the squared
-// deviations of pass B (kStatB), both reduced on the device and copied to
-// host_out (kPStat + kStatB doubles) by one copy, all on stream s: no host
-// round trip between the two passes.  Returns without waiting.
+// The PrintStats sums (kPStat) and the squared deviations of p, T and 1/rho
+// about their means (kStatB slots), from one sweep over the state, reduced on
+// the device and copied to host_out (kPStat + kStatB doubles) by one copy,
+// all on stream s.  Returns without waiting.
 hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host_out) {
     const StatPar p = stat_par(h, box);
     const size_t off = (size_t)box * h->ncell;
@@ -766,17 +807,8 @@ hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host
         }
         nb = launch_print_stats_st(h, st, p, s);
     }
-    hipLaunchKernelGGL((k_reduce_final<kPStat, 0u>), dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part, h->red_out);
-    // pass B about p, T and 1/rho means (slots 12..14; rho's slot is not reported)
-    MeanSrc ms;
-    ms.sums = h->red_out;
-    ms.slot[0] = -1;
-    ms.slot[1] = 12;
-    ms.slot[2] = 13;
-    ms.slot[3] = 14;
-    ms.N = (double)h->grid.nx * h->grid.ny * h->grid.nz;
-    const double m0[4] = {0.0, 0.0, 0.0, 0.0};
-    launch_stats_b_dev(h, box, m0, ms, s, h->red_out + kPStat);
+    hipLaunchKernelGGL(k_pstats_final, dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part,
+                       (double)h->grid.nx * h->grid.ny * h->grid.nz, h->red_out);
     return hipMemcpyAsync(host_out, h->red_out, sizeof(double) * (kPStat + kStatB), hipMemcpyDeviceToHost, s);
 }
 
