@@ -98,6 +98,7 @@ void free_ctx(Ctx* h) {
     if (h->halo_recv) (void)hipFree(h->halo_recv);
     if (h->spl_grad) (void)hipFree(h->spl_grad);
     if (h->spl_inv) (void)hipFree(h->spl_inv);
+    if (h->spl_wt) (void)hipFree(h->spl_wt);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->ps_host) (void)hipHostFree(h->ps_host);
     if (h->arena) (void)hipFree(h->arena);
@@ -731,8 +732,9 @@ int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
 
 int gkm_stats(gkm_t* h, gkm_stats_t* out) { return gkm_stats_box(h, 0, out); }
 
-// the nine gradient planes of one box and the Thomas pivots of the three
-// line lengths, allocated on first use
+// the nine gradient planes of one box, the Thomas pivots of the three line
+// lengths and the backward-sum weights of the y and x lines, allocated on
+// first use
 static int ensure_spline_buffers(Ctx* h) {
     if (h->spl_grad) return GKM_OK;
     const gkm::Geo& g = h->geo;
@@ -746,6 +748,25 @@ static int ensure_spline_buffers(Ctx* h) {
             inv.push_back(prev);
         }
     }
+    // D_i = sum_j W[i][j] y_{i+j}: W[i][j] = (-1)^j inv_i ... inv_{i+j-1} for
+    // i + j <= m - 1 (D_{m-1} = y_{m-1}), 0 beyond; rows of gkm::kSplRing
+    constexpr int R = gkm::kSplRing;
+    std::vector<double> wt;
+    size_t first = 0;
+    for (int m : {g.P0, g.P1}) {
+        for (int i = 0; i < m; i++) {
+            double w = 1.0;
+            for (int j = 0; j < R; j++) {
+                if (i + j <= m - 1) {
+                    wt.push_back(w);
+                    w *= -inv[first + i + j];
+                } else {
+                    wt.push_back(0.0);
+                }
+            }
+        }
+        first += (size_t)m;
+    }
     hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
     if (e != hipSuccess) {
         h->spl_grad = nullptr;
@@ -754,6 +775,8 @@ static int ensure_spline_buffers(Ctx* h) {
     }
     HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
     HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc((void**)&h->spl_wt, sizeof(double) * wt.size()));
+    HIP_TRY(hipMemcpy(h->spl_wt, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice));
     return GKM_OK;
 }
 

@@ -75,6 +75,7 @@ struct Ctx {
     // of the three line lengths (P0, P1, P2 values), allocated on first use
     double* spl_grad = nullptr;
     double* spl_inv = nullptr;
+    double* spl_wt = nullptr;  // backward-sum weights of the y (P0 rows) and x (P1 rows) lines, 32 per row
     // gkm_step_print_stats: a second stream (default priority) for the
     // statistics, its events and the pinned host rows of the sums
     hipStream_t stream2 = nullptr;
@@ -118,6 +119,7 @@ constexpr int kStatB = 4;
 hipError_t launch_stats_a(Ctx* h, int box, double* host_out /*kStatA*/);
 hipError_t launch_stats_b(Ctx* h, int box, const double means[4], double* host_out /*kStatB*/);
 constexpr int kPStat = 15;  // PrintStats sums
+constexpr int kSplRing = 32;  // terms of the backward sums of the x and y line slopes
 // spline slopes into h->spl_grad (allocated by the caller), then the sums
 hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host_out /*kPStat + kStatB*/);
 

@@ -271,18 +271,16 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
 //   D_{m-2} + 2 D_{m-1} = (3/h)(f_{m-1} - f_{m-2})
 // The matrix is the same for every line of an axis, so the Thomas pivots
 // inv_i = 1/(b_i - inv_{i-1}) come precomputed: forward y_i = (r_i - y_{i-1})
-// inv_i, backward D_i = y_i - inv_i D_{i+1}.  The forward values of nodes
-// 3..m-1 go to the axis's gradient plane ((3 A + comp) * ncell; A = 0 y,
-// 1 x, 2 z) and are overwritten by the slopes on the way back; only interior
-// nodes' slopes are used.
+// inv_i, backward D_i = y_i - inv_i D_{i+1}.  Only interior nodes' slopes are
+// used.
 //
 // Each axis is one pass that writes its slopes for the interior nodes to its
-// three planes; k_pstats then forms the sums from the nine planes and the
-// state.  Lines of up to kSplMaxM nodes (boxes up to 335 cells a side) are
-// solved whole in LDS (k_spl_lines, below); longer ones go in chunks through
-// the gradient planes (k_spl_chunked): blocks of 64 lines with three
-// wavefronts, wavefront q solving for velocity component q, so that a pass
-// over 256 x 256 lines puts three wavefronts on every SIMD.
+// three planes ((3 A + comp) * ncell; A = 0 y, 1 x, 2 z); k_pstats then forms
+// the sums from the nine planes and the state.  x and y lines are streamed,
+// a lane per line and component (k_spl_stream).  z lines, contiguous, are
+// solved whole in LDS when they have up to kSplMaxM nodes (boxes up to 335
+// cells a side, k_spl_lines); longer ones go in chunks through the gradient
+// planes (k_spl_chunked).
 // ---------------------------------------------------------------------------
 constexpr int kSpLines = 64, kSpThreads = 192, kSpChunk = 16;
 
@@ -336,79 +334,6 @@ __device__ __forceinline__ void pstat_add(const St& st, unsigned c, const StatPa
 // component q of the velocity at c
 template <class St> __device__ __forceinline__ double velq(const St& st, unsigned c, int q) {
     return st.get(1 + q, c) / st.get(0, c);
-}
-
-// The walks are latency-bound (one lane per line, each node a dependent
-// step), so they go in chunks of kSpChunk nodes: the chunk's loads are issued
-// together, then the recursion runs through them.
-
-// Forward walk of one line (base, stride, m nodes) for component q: y_i of
-// nodes 3..m-1 into plane G; returns y_{m-1} (= D_{m-1}).
-template <class St>
-__device__ __forceinline__ double spline_forward(const St& st, unsigned base, unsigned stride, int m, int q,
-                                                 const double* __restrict__ inv, double r3h, double* G) {
-    double f0 = velq(st, base, q), fm = f0, y = 0.0;
-    for (int i0 = 0; i0 < m; i0 += kSpChunk) {
-        double fn[kSpChunk];  // f at nodes i0+1 .. i0+kSpChunk
-#pragma unroll
-        for (int k = 0; k < kSpChunk; k++) {
-            const int n = i0 + 1 + k;
-            fn[k] = n < m ? velq(st, base + (unsigned)n * stride, q) : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < kSpChunk; k++) {
-            const int i = i0 + k;
-            if (i < m) {
-                // one-sided differences at the two ends, central inside
-                const double hi = i == m - 1 ? f0 : fn[k];
-                const double lo = i == 0 ? f0 : fm;
-                y = (r3h * (hi - lo) - y) * inv[i];
-                if (i >= 3) put(G, base + (unsigned)i * stride, y);
-                fm = f0;
-                f0 = fn[k];
-            }
-        }
-    }
-    return y;
-}
-
-// Backward walk from D_{m-1} = d over nodes m-2 .. 3 of the same line: the
-// slopes overwrite the forward values in G.
-__device__ __forceinline__ void spline_backward(unsigned base, unsigned stride, int m,
-                                                const double* __restrict__ inv, double d, double* G) {
-    for (int i1 = m - 2; i1 >= 3; i1 -= kSpChunk) {
-        double yv[kSpChunk];  // forward values of nodes i1, i1-1, ...
-#pragma unroll
-        for (int k = 0; k < kSpChunk; k++) {
-            const int n = i1 - k;
-            yv[k] = n >= 3 ? ld(G, base + (unsigned)n * stride) : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < kSpChunk; k++) {
-            const int n = i1 - k;
-            if (n >= 3) {
-                d = yv[k] - inv[n] * d;
-                put(G, base + (unsigned)n * stride, d);
-            }
-        }
-    }
-}
-
-// line (b, lane) of a strided axis (A = 0: y lines over (x, z); A = 1: x
-// lines over (y, z); z inner) for component q
-template <class St, int A>
-__device__ __forceinline__ void spline_strided_block(const Geo& g, const St& st, const double* __restrict__ inv,
-                                                     double r3h, double* __restrict__ grad, int b) {
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const unsigned n_out = A == 0 ? g.nx : g.ny;
-    const unsigned t = (unsigned)b * kSpLines + lane;
-    if (t >= n_out * g.nz) return;  // no barriers in this pass
-    const unsigned base = (t / g.nz + 3) * (A == 0 ? g.s1 : g.s0) + (t % g.nz + 3);
-    const unsigned stride = A == 0 ? g.s0 : g.s1;
-    const int m = A == 0 ? g.P0 : g.P1;
-    double* G = grad + (size_t)(3 * A + q) * g.ncell;
-    const double d = spline_forward(st, base, stride, m, q, inv, r3h, G);
-    spline_backward(base, stride, m, inv, d, G);
 }
 
 // z lines run along the unit stride, so a lane per line would touch a
@@ -497,34 +422,29 @@ __device__ __forceinline__ void spline_z_block(const Geo& g, const St& st, const
     }
 }
 
-// chunked pass over the lines of axis A: blocks of 64 lines x 3 components
-template <class St, int A>
+// chunked pass over the z lines: blocks of 64 lines x 3 components
+template <class St>
 __global__ void __launch_bounds__(kSpThreads) k_spl_chunked(Geo g, St st, const double* __restrict__ inv,
                                                             double r3h, double* __restrict__ grad) {
-    if constexpr (A == 2) {
-        __shared__ double tile[3][kSpLines][kSpChunk + 1];
-        spline_z_block(g, st, inv, r3h, grad, (int)blockIdx.x, tile);
-    } else {
-        spline_strided_block<St, A>(g, st, inv, r3h, grad, (int)blockIdx.x);
-    }
+    __shared__ double tile[3][kSpLines][kSpChunk + 1];
+    spline_z_block(g, st, inv, r3h, grad, (int)blockIdx.x, tile);
 }
 
-// ---- whole lines in LDS ------------------------------------------------------
-// For lines of up to kSplMaxM nodes (boxes up to 335 cells a side) a block
+// ---- whole z lines in LDS ----------------------------------------------------
+// For z lines of up to kSplMaxM nodes (boxes up to 335 cells a side) a block
 // holds a group of kSplL whole lines, all three components, in LDS:
 //   load:  every thread loads state and forms u, v, w at the group's nodes
 //   walk:  lane (component c, line l), 24 lanes, runs the forward and the
 //          backward recursion in place (f -> y -> D), out of LDS
 //   out:   the interior slopes go to the axis's planes.
-// Nothing but the state and the slopes crosses HBM (the chunked kernels
-// above also store and reload y), and every load and store covers a group's
-// lines at a node: 8 consecutive z for x and y lines, and runs along the
-// line for z lines, which are contiguous.  Measured at 256^3 (one MI355X):
-// about 370 us per axis against 450-580 us for the chunked passes.
+// Nothing but the state and the slopes crosses HBM (the chunked kernel above
+// also stores and reloads y), and every load and store runs along the lines,
+// which are contiguous.
 constexpr int kSplL = 8, kSplThreads = 192;
 constexpr int kSplMaxM = 341;  // 3 kSplL m doubles within 64 KB
 
-// line t of axis A (lines over the two other axes' interiors, the faster inner)
+// line t of axis A (lines over the two other axes' interiors, the faster
+// inner: z for the x and y lines)
 template <int A> struct SplAxis {
     unsigned n_in, n_out, stride;
     int m;
@@ -537,16 +457,13 @@ template <int A> struct SplAxis {
     }
 };
 
-// Elements (line l, node n) of a group as the 192 threads share them: for x
-// and y lines thread t keeps line t % 8 and takes nodes t / 8 + 24 k (a
-// node's 8 lines are 8 consecutive z); for z lines, contiguous, thread t
-// takes node t + 192 k of all 8 lines.
+// Elements (line l, node n) of a group as the 192 threads share them: thread
+// t takes node t + 192 k of all 8 lines.
 //
 // spl_load: the velocities of the group's nodes into the tile.  A batch's
-// loads (8 elements) are all issued before its divisions and LDS writes, with
-// no branch around an element: a missing line (the last group's) repeats the
-// group's last line, a node past m - 1 repeats node m - 1; their slots get the
-// same values again or are never read.
+// loads (one node of the 8 lines) are all issued before its divisions and LDS
+// writes, with no branch around an element: a missing line (the last group's)
+// repeats the group's last line, whose slot is never read.
 template <class St, int A>
 __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAxis<A>& ax, unsigned t0,
                                          unsigned nl, double* tile) {
@@ -570,7 +487,8 @@ __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAx
         tile[(1 * m + n) * kSplL + l] = v[2] / v[0];
         tile[(2 * m + n) * kSplL + l] = v[3] / v[0];
     };
-    if (A == 2) {
+    {
+        static_assert(A == 2, "x and y lines go through k_spl_stream");
         static_assert(K == kSplL, "one batch = one node of the 8 lines");
         unsigned b[kSplL];
 #pragma unroll
@@ -589,25 +507,6 @@ __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAx
                 put_f(l, n, v);
             }
         }
-    } else {
-        const int l = t % kSplL, step = kSplThreads / kSplL;
-        const unsigned b = ax.base(g, min(t0 + l, nl - 1));
-        for (int n = t / kSplL; n < m; n += K * step) {
-            Raw h[K][4], lo[K][4];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const unsigned c = b + (unsigned)min(n + k * step, m - 1) * ax.stride;
-#pragma unroll
-                for (int q = 0; q < 4; q++) raw(q, c, h[k][q], lo[k][q]);
-            }
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                double v[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) v[q] = (double)h[k][q] + (double)lo[k][q];
-                put_f(l, min(n + k * step, m - 1), v);
-            }
-        }
     }
 }
 
@@ -615,18 +514,12 @@ __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAx
 template <int A, class F>
 __device__ __forceinline__ void spl_store(const Geo& g, const SplAxis<A>& ax, unsigned t0, unsigned nl, int n0,
                                           int n1, F&& body) {
+    static_assert(A == 2, "x and y lines go through k_spl_stream");
     const int t = threadIdx.x;
-    if (A == 2) {
-        for (int l = 0; l < kSplL; l++) {
-            if (t0 + l >= nl) break;
-            const unsigned b = ax.base(g, t0 + l);
-            for (int n = n0 + t; n < n1; n += kSplThreads) body(l, n, b + (unsigned)n);
-        }
-    } else {
-        const int l = t % kSplL;
-        if (t0 + l >= nl) return;
+    for (int l = 0; l < kSplL; l++) {
+        if (t0 + l >= nl) break;
         const unsigned b = ax.base(g, t0 + l);
-        for (int n = n0 + t / kSplL; n < n1; n += kSplThreads / kSplL) body(l, n, b + (unsigned)n * ax.stride);
+        for (int n = n0 + t; n < n1; n += kSplThreads) body(l, n, b + (unsigned)n);
     }
 }
 
@@ -688,6 +581,94 @@ __device__ __forceinline__ void spl_group_solve(const Geo& g, const St& st, cons
         }
     }
     __syncthreads();
+}
+
+// x (A = 1) and y (A = 0) lines by one streaming sweep per line and velocity
+// component, with no LDS tile.  The forward recursion runs node by node as in
+// the whole-line walk (the same expressions, so the same y).  The backward one
+// is unrolled: D_i = y_i - inv_i D_{i+1} with D_{m-1} = y_{m-1} gives
+//   D_i = sum_j W[i][j] y_{i+j},  W[i][j] = (-1)^j inv_i ... inv_{i+j-1}
+// (j up to m-1-i), weights made on the host.  Past the first nodes the pivots
+// are 2 - sqrt(3) = 0.268, so W[i][j] falls like 0.268^j, and the sum is cut
+// at j < kSplRing = 32 (0.268^32 = 5e-19 of the leading term, below FP64
+// rounding); near the line's end it is whole.  So D_i is ready once y_{i+31}
+// is: a lane keeps the last 32 y in registers and writes D_i then.  Lanes run
+// along z, so every load and store is a full row; wavefront q of a block
+// takes velocity component q of the block's 64 lines.  The values differ from
+// the two-walk ones by rounding only.
+constexpr int kSplAhead = 4;
+template <class St> struct SplRaw;  // raw words of rho and rho u_q at one node
+template <> struct SplRaw<StateM> {
+    float h0, l0, hq, lq;
+    __device__ __forceinline__ void load(const StateM& st, int q, unsigned c) {
+        h0 = ld(st.hi[0], c);
+        l0 = ld(st.lo[0], c);
+        hq = ld(st.hi[1 + q], c);
+        lq = ld(st.lo[1 + q], c);
+    }
+    __device__ __forceinline__ double f() const { return ((double)hq + (double)lq) / ((double)h0 + (double)l0); }
+};
+template <> struct SplRaw<StateD> {
+    double w0, wq;
+    __device__ __forceinline__ void load(const StateD& st, int q, unsigned c) {
+        w0 = ld(st.w[0], c);
+        wq = ld(st.w[1 + q], c);
+    }
+    __device__ __forceinline__ double f() const { return wq / w0; }
+};
+
+template <class St, int A>
+__global__ void __launch_bounds__(192) k_spl_stream(Geo g, St st, const double* __restrict__ inv,
+                                                   const double* __restrict__ wt, double r3h,
+                                                   double* __restrict__ grad) {
+    static_assert(A != 2, "z lines go through k_spl_lines");
+    static_assert(kSplRing % kSplAhead == 0, "queue slots fixed within the unrolled ring");
+    const SplAxis<A> ax(g);
+    const unsigned nl = ax.n_in * ax.n_out;
+    const unsigned t = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // this wavefront's component
+    if (t >= nl) return;
+    const int m = ax.m;
+    const unsigned b = ax.base(g, t), s = ax.stride;
+    double* G = grad + (size_t)(3 * A + q) * g.ncell;
+    // raw words of nodes n+1 .. n+kSplAhead, node k in slot k % kSplAhead
+    SplRaw<St> Q[kSplAhead];
+    SplRaw<St> r0;
+    r0.load(st, q, b);
+#pragma unroll
+    for (int k = 1; k <= kSplAhead; k++) Q[k % kSplAhead].load(st, q, b + (unsigned)min(k, m - 1) * s);
+    double fm = r0.f(), f0 = fm, fp = fm, y = 0.0;  // f_{n-1}, f_n, f_{n+1}; y_{-1} = 0
+    double ring[kSplRing];  // y_k in slot k % kSplRing
+#pragma unroll
+    for (int j = 0; j < kSplRing; j++) ring[j] = 0.0;
+    for (int n0 = 0; n0 <= m + kSplRing - 5; n0 += kSplRing) {
+#pragma unroll
+        for (int u = 0; u < kSplRing; u++) {
+            const int n = n0 + u;
+            if (n < m) {
+                if (n + 1 < m) {
+                    fp = Q[(u + 1) % kSplAhead].f();
+                    const int k = n + 1 + kSplAhead;
+                    if (k < m) Q[(u + 1) % kSplAhead].load(st, q, b + (unsigned)k * s);
+                }
+                // one-sided differences at the two ends, central inside
+                const double hi = n == m - 1 ? f0 : fp;
+                const double lo = n == 0 ? f0 : fm;
+                y = (r3h * (hi - lo) - y) * inv[n];
+                ring[u] = y;
+                fm = f0;
+                f0 = fp;
+            }
+            const int i = n - (kSplRing - 1);  // the node whose window is now complete
+            if (i >= 3 && i <= m - 4) {
+                const double* w = wt + (size_t)i * kSplRing;
+                double d = 0.0;
+#pragma unroll
+                for (int j = kSplRing - 1; j >= 0; j--) d = fma(w[j], ring[(u + 1 + j) % kSplRing], d);
+                put(G, b + (unsigned)i * s, d);
+            }
+        }
+    }
 }
 
 // lines of axis A: interior slopes into planes 3 A .. 3 A + 2
@@ -770,18 +751,17 @@ static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStre
     const int m[3] = {g.P0, g.P1, g.P2};
     const unsigned nl[3] = {(unsigned)g.nx * g.nz, (unsigned)g.ny * g.nz, (unsigned)g.ny * g.nx};
     auto tile_bytes = [](int mm) { return sizeof(double) * 3 * (size_t)mm * kSplL; };
-    auto axis = [&](auto Ax) {
-        constexpr int A = decltype(Ax)::value;
-        if (m[A] <= kSplMaxM)
-            hipLaunchKernelGGL((k_spl_lines<St, A>), dim3((nl[A] + kSplL - 1) / kSplL), dim3(kSplThreads),
-                               tile_bytes(m[A]), s, g, st, inv[A], r3h, h->spl_grad);
-        else
-            hipLaunchKernelGGL((k_spl_chunked<St, A>), dim3((nl[A] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0,
-                               s, g, st, inv[A], r3h, h->spl_grad);
-    };
-    axis(std::integral_constant<int, 2>());
-    axis(std::integral_constant<int, 1>());
-    axis(std::integral_constant<int, 0>());
+    if (m[2] <= kSplMaxM)
+        hipLaunchKernelGGL((k_spl_lines<St, 2>), dim3((nl[2] + kSplL - 1) / kSplL), dim3(kSplThreads),
+                           tile_bytes(m[2]), s, g, st, inv[2], r3h, h->spl_grad);
+    else
+        hipLaunchKernelGGL((k_spl_chunked<St>), dim3((nl[2] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0, s, g,
+                           st, inv[2], r3h, h->spl_grad);
+    const unsigned sb[2] = {(nl[0] + 63) / 64, (nl[1] + 63) / 64};
+    hipLaunchKernelGGL((k_spl_stream<St, 1>), dim3(sb[1]), dim3(192), 0, s, g, st, inv[1],
+                       h->spl_wt + (size_t)g.P0 * kSplRing, r3h, h->spl_grad);
+    hipLaunchKernelGGL((k_spl_stream<St, 0>), dim3(sb[0]), dim3(192), 0, s, g, st, inv[0], h->spl_wt, r3h,
+                       h->spl_grad);
     hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, s, g, st, h->spl_grad, p,
                        make_fastdiv((unsigned)g.nz), make_fastdiv((unsigned)g.nx), h->red_part);
     return h->red_blocks;
