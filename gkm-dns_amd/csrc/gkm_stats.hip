@@ -597,6 +597,7 @@ __device__ __forceinline__ void spl_group_solve(const Geo& g, const St& st, cons
 // takes velocity component q of the block's 64 lines.  The values differ from
 // the two-walk ones by rounding only.
 constexpr int kSplAhead = 4;
+constexpr int kSplStreamThreads = 192;  // 64 lines x 3 velocity components
 template <class St> struct SplRaw;  // raw words of rho and rho u_q at one node
 template <> struct SplRaw<StateM> {
     float h0, l0, hq, lq;
@@ -618,7 +619,7 @@ template <> struct SplRaw<StateD> {
 };
 
 template <class St, int A>
-__global__ void __launch_bounds__(192) k_spl_stream(Geo g, St st, const double* __restrict__ inv,
+__global__ void __launch_bounds__(kSplStreamThreads) k_spl_stream(Geo g, St st, const double* __restrict__ inv,
                                                    const double* __restrict__ wt, double r3h,
                                                    double* __restrict__ grad) {
     static_assert(A != 2, "z lines go through k_spl_lines");
@@ -758,9 +759,10 @@ static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStre
         hipLaunchKernelGGL((k_spl_chunked<St>), dim3((nl[2] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0, s, g,
                            st, inv[2], r3h, h->spl_grad);
     const unsigned sb[2] = {(nl[0] + 63) / 64, (nl[1] + 63) / 64};
-    hipLaunchKernelGGL((k_spl_stream<St, 1>), dim3(sb[1]), dim3(192), 0, s, g, st, inv[1],
+    const dim3 sth(kSplStreamThreads);
+    hipLaunchKernelGGL((k_spl_stream<St, 1>), dim3(sb[1]), sth, 0, s, g, st, inv[1],
                        h->spl_wt + (size_t)g.P0 * kSplRing, r3h, h->spl_grad);
-    hipLaunchKernelGGL((k_spl_stream<St, 0>), dim3(sb[0]), dim3(192), 0, s, g, st, inv[0], h->spl_wt, r3h,
+    hipLaunchKernelGGL((k_spl_stream<St, 0>), dim3(sb[0]), sth, 0, s, g, st, inv[0], h->spl_wt, r3h,
                        h->spl_grad);
     hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, s, g, st, h->spl_grad, p,
                        make_fastdiv((unsigned)g.nz), make_fastdiv((unsigned)g.nx), h->red_part);
