@@ -15,13 +15,21 @@
  *        compiled reference flux() on random and jump states
  *        (tests/golden/flux_golden.npz) plus closed forms (uniform state ->
  *        dt x Euler flux; Gaussian moments; inverse identity at Pr = 1).
- *   orc_weno / orc_weno_scalar -> closed forms (constant data, quadratic
- *        reproduction, smooth limit = 6-point central / 5th-order upwind
- *        interpolants, mirror symmetry) and brute-force recomputation of the
- *        nonlinear weights on tiny inputs.
+ *   orc_weno / orc_weno_scalar -> an independent rebuild of SYMOO and JS from
+ *        their definitions (candidate quadratics solved from the cell-average
+ *        integrals, smoothness indicators integrated exactly over the upwind
+ *        cell, the published ideal weights and the downwind maximum) on rough,
+ *        step, spike and linear lines (tests/test_weno_pin.py), plus closed
+ *        forms (constant data, quadratic reproduction, smooth limit, mirror
+ *        symmetry).
  *   orc_step -> telescoping conservation, uniform state fixed bitwise,
  *        X/Y axis-permutation symmetry (Z too with zbug_compat = 0), and
  *        composition of the pinned parts on single interfaces.
+ *   orc_stats -> closed-form fields (Fourier modes with their exact central
+ *        differences, Taylor-Green cells without dilatation, a density wave,
+ *        a skewed two-mode line) in tests/test_stats_pin.py.
+ *   orc_spline_slopes / orc_print_stats -> scipy's natural CubicSpline and a
+ *        numpy restatement of PrintStats (tests/test_print_stats.py).
  */
 #include "gkm_oracle.h"
 
