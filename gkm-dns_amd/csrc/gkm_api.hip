@@ -101,6 +101,8 @@ void free_ctx(Ctx* h) {
     if (h->spl_wt) (void)hipFree(h->spl_wt);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->ps_host) (void)hipHostFree(h->ps_host);
+    if (h->h_bad) (void)hipHostFree(h->h_bad);
+    if (h->ev_bad) (void)hipEventDestroy(h->ev_bad);
     if (h->arena) (void)hipFree(h->arena);
     if (h->ev_state) (void)hipEventDestroy(h->ev_state);
     for (hipEvent_t e : h->ev_ps)
@@ -108,6 +110,45 @@ void free_ctx(Ctx* h) {
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+// GKM_E_NONPHYSICAL for the first box whose flag h_bad holds
+int bad_report(const Ctx* h) {
+    for (int b = 0; b < h->nbox; b++)
+        if (h->h_bad[b])
+            return fail(GKM_E_NONPHYSICAL,
+                        "box %d: a step produced a non-physical state (rho <= 0, p <= 0 or a non-finite value); "
+                        "replace it with gkm_set_state",
+                        b);
+    return GKM_OK;
+}
+
+// wait for the handle's stream, then read the non-physical flags
+int check_physical(Ctx* h) {
+    HIP_TRY(hipMemcpyAsync(h->h_bad, h->d_bad, sizeof(unsigned) * h->nbox, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return bad_report(h);
+}
+
+// the flags of the copy that ended the last stepping call, if it has landed
+// (no wait); a copy still in flight is read by a later call
+int poll_physical(Ctx* h) {
+    if (!h->bad_copied) return GKM_OK;
+    const hipError_t q = hipEventQuery(h->ev_bad);
+    if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear it from the thread's last-error state
+        return GKM_OK;
+    }
+    if (q != hipSuccess) return fail(GKM_E_HIP, "hipEventQuery failed: %s", hipGetErrorString(q));
+    return bad_report(h);
+}
+
+// queue the copy of the flags that poll_physical reads
+int queue_flag_copy(Ctx* h) {
+    HIP_TRY(hipMemcpyAsync(h->h_bad, h->d_bad, sizeof(unsigned) * h->nbox, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipEventRecord(h->ev_bad, h->stream));
+    h->bad_copied = true;
+    return GKM_OK;
 }
 
 }  // namespace
@@ -178,9 +219,12 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     h->red_blocks = 2048;
     const size_t red = up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA) + up(sizeof(double) * 64);
     const size_t pars = up(sizeof(gkm::StepPar) * nbox);
-    h->arena_bytes = plane * nplanes + red + pars;
+    const size_t flags = up(sizeof(unsigned) * nbox);
+    h->arena_bytes = plane * nplanes + red + pars + flags;
     e = hipMalloc(&h->arena, h->arena_bytes);
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_pars, sizeof(gkm::StepPar) * nbox, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_bad, sizeof(unsigned) * nbox, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_bad, hipEventDisableTiming);
     if (e != hipSuccess) {
         const size_t bytes = h->arena_bytes;
         free_ctx(h);
@@ -214,6 +258,8 @@ int gkm_create_batch(const gkm_grid* grid, const gkm_params* params, int nbox, g
     h->red_part = (double*)take(up(sizeof(double) * (size_t)h->red_blocks * gkm::kStatA));
     h->red_out = (double*)take(up(sizeof(double) * 64));
     h->d_pars = (gkm::StepPar*)take(pars);
+    h->d_bad = (unsigned*)take(flags);
+    std::memset(h->h_bad, 0, sizeof(unsigned) * nbox);
     // interface and flux planes start at zero so that unused border entries are defined
     e = hipMemsetAsync(h->arena, 0, h->arena_bytes, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -294,6 +340,7 @@ int gkm_set_state_box(gkm_t* hh, int box, const double* const W[5]) {
     const size_t bytes = sizeof(double) * (size_t)h->ncell;
     const size_t off = (size_t)box * h->ncell;
     h->derived_valid = false;
+    HIP_TRY(hipMemsetAsync(h->d_bad + box, 0, sizeof(unsigned), h->stream));
     if (h->par.precision == GKM_PREC_FP64) {
         for (int q = 0; q < 5; q++)
             HIP_TRY(hipMemcpyAsync(h->Wd[q] + off, W[q], bytes, hipMemcpyHostToDevice, h->stream));
@@ -306,6 +353,7 @@ int gkm_set_state_box(gkm_t* hh, int box, const double* const W[5]) {
         }
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
+    h->h_bad[box] = 0;  // every flag copy queued before has landed
     return GKM_OK;
 }
 
@@ -349,7 +397,9 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
     for (int b = 0; b < h->nbox; b++)
         if (!(dt[b] > 0) || !std::isfinite(dt[b])) return fail(GKM_E_ARG, "dt[%d] must be positive and finite", b);
     if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
-    int rc = upload_pars(h, dt);
+    int rc = poll_physical(h);
+    if (rc) return rc;
+    rc = upload_pars(h, dt);
     if (rc) return rc;
     if (h->timing && !h->events_made) {
         for (auto& ev : h->ev_step) HIP_TRY(hipEventCreate(&ev));
@@ -373,19 +423,31 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
                 HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
                 gkm::launch_step(h, nullptr);  // swaps the pointers for the work it records
                 hipGraph_t gr = nullptr;
-                HIP_TRY(hipStreamEndCapture(h->stream, &gr));
+                hipError_t e = hipStreamEndCapture(h->stream, &gr);
+                if (e == hipSuccess) e = hipGraphInstantiate(&h->graph_exec[f], gr, nullptr, nullptr, 0);
+                if (e == hipSuccess) e = hipGraphLaunch(h->graph_exec[f], h->stream);
+                if (e != hipSuccess) {
+                    // the recorded step never ran: the current state is still the one it started from
+                    if (h->graph_exec[f]) (void)hipGraphExecDestroy(h->graph_exec[f]);
+                    if (gr) (void)hipGraphDestroy(gr);
+                    h->graph_exec[f] = nullptr;
+                    gkm::swap_state(h);
+                    h->derived_valid = false;
+                    return fail(GKM_E_HIP, "captured step failed: %s", hipGetErrorString(e));
+                }
                 h->graph[f] = gr;
-                HIP_TRY(hipGraphInstantiate(&h->graph_exec[f], h->graph[f], nullptr, nullptr, 0));
                 h->graph_ready[f] = true;
+            } else {
+                HIP_TRY(hipGraphLaunch(h->graph_exec[f], h->stream));
+                gkm::swap_state(h);
             }
-            HIP_TRY(hipGraphLaunch(h->graph_exec[f], h->stream));
-            if (!capture) gkm::swap_state(h);
         } else {
             gkm::launch_step(h, nullptr);
         }
     }
     gkm::ensure_ghosts(h);
     HIP_TRY(hipGetLastError());
+    if (nsteps > 0) return queue_flag_copy(h);
     return GKM_OK;
 }
 
@@ -423,6 +485,7 @@ int gkm_set_tuning(gkm_t* hh, int flux_block, int recon_chunk) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->flux_block = flux_block;
     h->recon_chunk = recon_chunk;
+    h->tune_gen++;  // groups holding the handle capture their step again
     drop_graph(h);
     return GKM_OK;
 }
@@ -430,8 +493,7 @@ int gkm_set_tuning(gkm_t* hh, int flux_block, int recon_chunk) {
 int gkm_sync(gkm_t* hh) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return GKM_OK;
+    return check_physical(h);
 }
 
 // ---- groups ----------------------------------------------------------------
@@ -450,13 +512,19 @@ struct gkm_group {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<Cap> caps;
+    std::vector<unsigned> gens;  // the handles' tuning generations the captures were made with
 };
 
-static void free_group(gkm_group* g) {
+static void drop_group_graphs(gkm_group* g) {
     for (auto& c : g->caps) {
         if (c.exec) (void)hipGraphExecDestroy(c.exec);
         if (c.graph) (void)hipGraphDestroy(c.graph);
     }
+    g->caps.clear();
+}
+
+static void free_group(gkm_group* g) {
+    drop_group_graphs(g);
     for (auto e : g->pre)
         if (e) (void)hipEventDestroy(e);
     for (auto e : g->join)
@@ -507,8 +575,18 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
     if (!g) return fail(GKM_E_ARG, "null group");
     if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
     const int n = (int)g->hs.size();
+    // launch shapes changed since the captures (gkm_set_tuning): capture again
+    std::vector<unsigned> gens(n);
+    for (int i = 0; i < n; i++) gens[i] = g->hs[i]->tune_gen;
+    if (gens != g->gens) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        drop_group_graphs(g);
+        g->gens = gens;
+    }
     for (int i = 0; i < n; i++) {
         Ctx* h = g->hs[i];
+        int rc = poll_physical(h);
+        if (rc) return rc;
         // every box needs a dt; constants changed since the last upload are
         // uploaded with the dt values last given
         std::vector<double> dt(h->nbox);
@@ -547,14 +625,20 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
                 (void)hipEventRecord(g->join[i], h->stream);
                 (void)hipStreamWaitEvent(g->stream, g->join[i], 0);
             }
-            HIP_TRY(hipStreamEndCapture(g->stream, &c.graph));
-            hipError_t e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+            hipError_t e = hipStreamEndCapture(g->stream, &c.graph);
+            if (e == hipSuccess) e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+            if (e == hipSuccess) e = hipGraphLaunch(c.exec, g->stream);
             if (e != hipSuccess) {
-                (void)hipGraphDestroy(c.graph);
-                return fail(GKM_E_HIP, "group graph instantiation failed: %s", hipGetErrorString(e));
+                // the recorded steps never ran: every handle keeps the state it started from
+                if (c.exec) (void)hipGraphExecDestroy(c.exec);
+                if (c.graph) (void)hipGraphDestroy(c.graph);
+                for (Ctx* h : g->hs) {
+                    gkm::swap_state(h);
+                    h->derived_valid = false;
+                }
+                return fail(GKM_E_HIP, "group step capture failed: %s", hipGetErrorString(e));
             }
             g->caps.push_back(c);
-            HIP_TRY(hipGraphLaunch(g->caps.back().exec, g->stream));
         } else {
             HIP_TRY(hipGraphLaunch(cap->exec, g->stream));
             for (Ctx* h : g->hs) gkm::swap_state(h);
@@ -565,6 +649,10 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
     for (Ctx* h : g->hs) {
         HIP_TRY(hipStreamWaitEvent(h->stream, g->done, 0));
         gkm::ensure_ghosts(h);
+        if (nsteps > 0) {
+            int rc = queue_flag_copy(h);
+            if (rc) return rc;
+        }
     }
     HIP_TRY(hipGetLastError());
     return GKM_OK;
@@ -573,7 +661,10 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
 int gkm_group_sync(gkm_group_t* g) {
     if (!g) return fail(GKM_E_ARG, "null group");
     HIP_TRY(hipStreamSynchronize(g->stream));
-    for (Ctx* h : g->hs) HIP_TRY(hipStreamSynchronize(h->stream));
+    for (Ctx* h : g->hs) {
+        int rc = check_physical(h);
+        if (rc) return rc;
+    }
     return GKM_OK;
 }
 
@@ -668,6 +759,8 @@ int gkm_compute_dt_box(gkm_t* hh, int box, double cfl, double* dt) {
     if (!(cfl > 0)) return fail(GKM_E_ARG, "cfl must be positive");
     double a[gkm::kStatA];
     HIP_TRY(gkm::launch_stats_a(h, box, a));
+    rc = check_physical(h);
+    if (rc) return rc;
     const double mx = a[27];
     if (!(mx > 0) || !std::isfinite(mx)) return fail(GKM_E_NONPHYSICAL, "signal speed %g is not positive/finite", mx);
     *dt = cfl * h->grid.dx / mx;
@@ -725,6 +818,8 @@ int gkm_stats_box(gkm_t* hh, int box, gkm_stats_t* out) {
     s.T_rms = std::sqrt(b[2] / N);
     s.v_rms = std::sqrt(b[3] / N);
     *out = s;
+    rc = check_physical(h);
+    if (rc) return rc;
     if (!(s.rho_mean > 0) || !(s.p_mean > 0) || !std::isfinite(s.energy))
         return fail(GKM_E_NONPHYSICAL, "non-physical state: <rho>=%g <p>=%g", s.rho_mean, s.p_mean);
     return GKM_OK;
@@ -823,8 +918,10 @@ int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
     gkm::ensure_ghosts(h);
     double a[gkm::kPStat + gkm::kStatB];
     HIP_TRY(gkm::launch_print_stats_async(h, box, h->stream, a));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return fill_print_stats(h, a, a + gkm::kPStat, out);
+    HIP_TRY(hipStreamSynchronize(h->stream));  // the sums land in a
+    rc = check_physical(h);
+    const int rs = fill_print_stats(h, a, a + gkm::kPStat, out);
+    return rc ? rc : rs;
 }
 
 // the loop of gkm_step_print_stats: statistics of state s on stream2 beside step s
@@ -893,7 +990,7 @@ int gkm_step_print_stats(gkm_t* hh, double dt, int nsteps, gkm_print_stats_t* ou
         rc = fill_print_stats(h, a, a + gkm::kPStat, out + s);
         if (rc) return rc;
     }
-    return GKM_OK;
+    return poll_physical(h);
 }
 
 int gkm_print_stats(gkm_t* h, gkm_print_stats_t* out) { return gkm_print_stats_box(h, 0, out); }

@@ -488,6 +488,82 @@ __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, co
     add_to_lab(F, m, s.V, s.W);
 }
 
+// Time-integral coefficients of the BGK solution (GKMflux.cpp:155-157,
+// 186-190) at x = dt/tau: gm1, gm2, gm4 of the time slope, and P, Q, p0..p2
+// (gm3 = -gm1, gm5 = tau gm3, p3 = P, p4 = -Q - tau P, p5 = -tau P follow).
+//
+// FP64: the reference's expressions, in its order.
+//
+// FP32: the same expressions lose every digit as x -> 0: gm0 = dt - tau(1 -
+// e^-x) = tau (x^2/2 - ...) is a difference of two O(dt) terms each rounded to
+// eps dt/x, so gm0 carries a relative error ~ 2 eps/x^2 (3e-4 at x = 0.16, the
+// reference's CFL 0.02 on a 256^3 box; 80 % at x = 0.01), and gm1, gm2, gm4,
+// gm5 divide by it.  Below x = 1/2 they are formed instead from the
+// exponential-integrator functions phi_k(-x) = sum_n (-x)^n / (n+k)!, which
+// have no cancellation (8 terms each: the first dropped term is < 3e-8 of
+// the sum at x = 1/2):
+//   gm1 = -phi1/(dt phi2)            gm2 = x (2 phi3 - phi2)/phi2
+//   gm4 = (phi2 - phi1)/phi2         P = dt phi1       Q = dt^2 (phi1 - phi2)
+//   p0 = dt x phi2   p1 = dt^2 x phi3   p2 = dt^2 x ((2 + x) phi3 - 1/2)
+// which are the reference's expressions rewritten with e^-x = 1 - x phi1,
+// phi1 = 1 - x phi2, phi2 = 1/2 - x phi3 (exact identities).  Worst relative
+// error of the ten coefficients in FP32: 2.8e-7 below x = 1/2, 1.9e-6 just
+// above it (the reference form).
+template <class T> struct TimeCoef {
+    T gm1, gm2, gm4, P, Q, p0, p1, p2;
+};
+
+__device__ __forceinline__ void time_coef(double dt, double tau, TimeCoef<double>& k) {
+    const double ex = M<double>::exp_(-dt * M<double>::rcp(tau));
+    const double om = 1.0 - ex;
+    const double gm0 = dt - tau * om;
+    const double igm0 = M<double>::rcp(gm0);
+    k.gm1 = -om * igm0;
+    k.gm2 = (-dt + 2 * tau * om - dt * ex) * igm0;
+    k.gm4 = (dt * ex - tau * om) * igm0;
+    k.P = tau * om;
+    k.Q = -tau * dt * ex + tau * tau * om;
+    k.p0 = dt - k.P;
+    k.p1 = 0.5 * dt * dt - tau * k.p0;
+    k.p2 = -tau * k.p0 + k.Q;
+}
+
+__device__ __forceinline__ void time_coef(float dt, float tau, TimeCoef<float>& k) {
+    const float x = dt * M<float>::rcp(tau);
+    if (x < 0.5f) {
+        // Horner in x of sum_{n<8} (-x)^n/(n+k)!
+        const float f1 = 1.f + x * (-1.f / 2 + x * (1.f / 6 + x * (-1.f / 24 + x * (1.f / 120 + x * (-1.f / 720 +
+                         x * (1.f / 5040 + x * (-1.f / 40320)))))));
+        const float f2 = 1.f / 2 + x * (-1.f / 6 + x * (1.f / 24 + x * (-1.f / 120 + x * (1.f / 720 +
+                         x * (-1.f / 5040 + x * (1.f / 40320 + x * (-1.f / 362880)))))));
+        const float f3 = 1.f / 6 + x * (-1.f / 24 + x * (1.f / 120 + x * (-1.f / 720 + x * (1.f / 5040 +
+                         x * (-1.f / 40320 + x * (1.f / 362880 + x * (-1.f / 3628800)))))));
+        const float if2 = M<float>::rcp(f2);
+        const float dt2 = dt * dt;
+        k.gm1 = -f1 * if2 * M<float>::rcp(dt);
+        k.gm2 = x * (2.f * f3 - f2) * if2;
+        k.gm4 = (f2 - f1) * if2;
+        k.P = dt * f1;
+        k.Q = dt2 * (f1 - f2);
+        k.p0 = dt * x * f2;
+        k.p1 = dt2 * x * f3;
+        k.p2 = dt2 * x * ((2.f + x) * f3 - 0.5f);
+        return;
+    }
+    const float ex = M<float>::exp_(-x);
+    const float om = 1.f - ex;
+    const float gm0 = dt - tau * om;
+    const float igm0 = M<float>::rcp(gm0);
+    k.gm1 = -om * igm0;
+    k.gm2 = (-dt + 2 * tau * om - dt * ex) * igm0;
+    k.gm4 = (dt * ex - tau * om) * igm0;
+    k.P = tau * om;
+    k.Q = -tau * dt * ex + tau * tau * om;
+    k.p0 = dt - k.P;
+    k.p1 = 0.5f * dt * dt - tau * k.p0;
+    k.p2 = -tau * k.p0 + k.Q;
+}
+
 // BGK-NS interface flux, time-integrated over dt (GKMflux.cpp:11-215).
 // All vectors are in the rotated slot order (slot 1 = normal momentum).
 // tau0 = mu/p of the left cell.  slopes(side, Dx, Dy, Dz) yields the slopes
@@ -504,20 +580,13 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     // collision time with the pressure-jump blending (GKMflux.cpp:149-150);
     // rho/lam = 2p, and the factor 2 cancels
     const T tau = tau0 + dt * fabs(l.P - r.P) * M<T>::rcp(fabs(l.P + r.P));
-    const T ex = M<T>::exp_(-dt * M<T>::rcp(tau));
-    const T om = T(1) - ex;
-    const T gm0 = dt - tau * om;
-    const T igm0 = M<T>::rcp(gm0);
-    const T gm1 = -om * igm0;
-    const T gm2 = (-dt + 2 * tau * om - dt * ex) * igm0;
+    TimeCoef<T> tc;
+    time_coef(dt, tau, tc);
+    const T gm1 = tc.gm1, gm2 = tc.gm2, gm4 = tc.gm4;
     const T gm3 = -gm1;
-    const T gm4 = (dt * ex - tau * om) * igm0;
     const T gm5 = tau * gm3;
-    const T P = tau * om;
-    const T Q = -tau * dt * ex + tau * tau * om;
-    const T p0 = dt - P;
-    const T p1 = T(0.5) * dt * dt - tau * p0;
-    const T p2 = -tau * p0 + Q;
+    const T P = tc.P, Q = tc.Q;
+    const T p0 = tc.p0, p1 = tc.p1, p2 = tc.p2;
     const T p3 = P;
     const T p4 = -Q - tau * P;
     const T p5 = -tau * P;

@@ -27,6 +27,7 @@ struct Ctx {
     int flux_block = 256;  // X and Y flux kernels (gkm_set_tuning)
     int zflux_block = 128;  // the fused Z kernel: 128 (mixed) or 64 (FP64), set at creation
     int recon_chunk = 8;
+    unsigned tune_gen = 0;  // counts gkm_set_tuning calls (group captures are keyed by it)
     // per-box step parameters: device array read by the kernels, pinned staging
     StepPar* d_pars = nullptr;
     StepPar* h_pars = nullptr;
@@ -64,6 +65,14 @@ struct Ctx {
     // or by a ghost fill); anything that changes the state or the gas
     // constants clears it
     bool derived_valid = false;
+    // non-physical flags, one per box, raised by the fused Z kernel on a new
+    // cell with rho or T not positive and finite; cleared by gkm_set_state_box.
+    // Every gkm_step_batch call ends with a copy of the flags to the pinned
+    // h_bad (event ev_bad), which the next call reads without waiting.
+    unsigned* d_bad = nullptr;
+    unsigned* h_bad = nullptr;
+    hipEvent_t ev_bad = nullptr;
+    bool bad_copied = false;
     // one device allocation holds every plane
     void* arena = nullptr;
     size_t arena_bytes = 0;

@@ -523,17 +523,35 @@ __device__ __forceinline__ unsigned image_mask(const Geo& g, unsigned J, unsigne
            (I >= (unsigned)g.nx ? 8u : 0u) | (K < 6u ? 16u : 0u) | (K >= (unsigned)g.nz ? 32u : 0u);
 }
 
+// A new cell value is physical when rho and T are positive and finite; T is
+// formed as cell_prims forms it (a non-finite momentum or energy makes T
+// non-finite or NaN, so this covers every field).  The fused Z kernel raises
+// the box's flag on any other value; the host reads it at its next wait.
+template <class St>
+__device__ __forceinline__ bool physical(const StepPar& par, const CellVal<St>& v) {
+    const double r0 = v.get(0);
+    const double m1 = v.get(1), m2 = v.get(2), m3 = v.get(3);
+    const double ri = 1.0 / r0;
+    const double T = (par.gam - 1) * (v.get(4) * ri - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * (ri * ri)) * par.ud[kUIR];
+    return r0 > 0.0 && r0 < __builtin_huge_val() && T > 0.0 && T < __builtin_huge_val();
+}
+
+// Returns, for the WENO-BGK scheme, whether the cell is physical (rho and the
+// Tw just formed positive and finite), and true for the GKS scheme.
 template <class Real, class St, int SCHEME>
-__device__ __forceinline__ void ghost_images(const Geo& g, unsigned imask, unsigned c, const St& N,
+__device__ __forceinline__ bool ghost_images(const Geo& g, unsigned imask, unsigned c, const St& N,
                                              const CellVal<St>& v, Real* Tw, Real* tau, const StepPar& par) {
     constexpr bool prim = SCHEME != GKM_SCHEME_GKS;
     Real Tc = Real(0), tc = Real(0);
+    bool ok = true;
     if constexpr (prim) {
         cell_prims<Real>(par, v.get(0), v.get(1), v.get(2), v.get(3), v.get(4), Tc, tc);
         put(Tw, c, Tc);
         put(tau, c, tc);
+        const double r0 = v.get(0);
+        ok = r0 > 0.0 && r0 < __builtin_huge_val() && Tc > Real(0) && Tc < Real(__builtin_huge_val());
     }
-    if (!imask) return;
+    if (!imask) return ok;
     const int n[3] = {g.ny * (int)g.s0, g.nx * (int)g.s1, g.nz};
     int off[3][3];  // per axis: own position, low image (+n), high image (-n); 0 = none
 #pragma unroll
@@ -557,6 +575,7 @@ __device__ __forceinline__ void ghost_images(const Geo& g, unsigned imask, unsig
             }
         }
     }
+    return ok;
 }
 
 // ---------------------------------------------------------------------------
@@ -600,7 +619,7 @@ template <class Real, class St, int SCHEME, bool GHOST>
 __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
                   Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars, Real* __restrict__ Tw_nx,
-                  Real* __restrict__ tau_nx) {
+                  Real* __restrict__ tau_nx, unsigned* __restrict__ bad) {
     const unsigned gt = xcd_block() * blockDim.x + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned e = (gt >> 6) * 63u + lane;
@@ -652,7 +671,14 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     if (!((upd_mask >> __lane_id()) & 1ull)) return;
     CellVal<St> v;
     update_finish(c, nx, Fz, Fm, upd_lds, par, v);
-    if constexpr (GHOST) ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
+    bool ok;
+    if constexpr (GHOST && SCHEME != GKM_SCHEME_GKS) {
+        ok = ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
+    } else {
+        if constexpr (GHOST) ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
+        ok = physical<St>(par, v);
+    }
+    if (!ok) bad[blockIdx.y] = 1u;
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -896,7 +922,7 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     auto fused = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(bs), lds, h->stream, g, zl, st1, st2, st, nx, tau, wi, F, h->d_pars,
-                           Tw_nx, tau_nx);
+                           Tw_nx, tau_nx, h->d_bad);
     };
     if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction
         if (ev) (void)hipEventRecord(ev[1], h->stream);

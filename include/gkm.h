@@ -194,7 +194,8 @@ int gkm_group_destroy(gkm_group_t* g);
 /* Launch shapes: threads per block of the X and Y flux kernels (64, 128 or
  * 256) and interfaces per thread of the strided-sweep WENO kernels (4, 8 or
  * 16).  Chosen per box size by the autotuner in gkm_amd/tune.py.  (The fused
- * Z sweep's block size is fixed per precision.) */
+ * Z sweep's block size is fixed per precision.)  Drops the handle's captured
+ * step; groups holding the handle capture theirs again on their next step. */
 int gkm_set_tuning(gkm_t* h, int flux_block, int recon_chunk);
 
 /* Copy the padded host state in / out (bit-exact round trip for FP64; the
@@ -212,7 +213,17 @@ int gkm_fill_ghosts(gkm_t* h);
  * flux sweeps and the conservative update; a final ghost fill leaves the
  * state as the reference leaves it after HaloTransfer.  Replaces
  * evolve(W, dev, dt) / evolveEulerian (GPUupdate.cu:65-196,
- * update.cpp:180-199).  Asynchronous on the handle's stream. */
+ * update.cpp:180-199).  Asynchronous on the handle's stream.
+ *
+ * Failure detection (the reference checks nothing, GPUupdate.cu:11-20): the
+ * update raises a per-box flag when a new cell has rho or T = p/(R rho) not
+ * positive and finite.  The flag is sticky until gkm_set_state(_box) replaces
+ * that box's state.  gkm_sync, gkm_compute_dt, gkm_stats, gkm_print_stats,
+ * gkm_step_print_stats and gkm_group_sync return GKM_E_NONPHYSICAL once the
+ * work that raised it has completed; gkm_step / gkm_step_batch /
+ * gkm_group_step return it without waiting, and without stepping, once the
+ * flag copy queued by an earlier stepping call has landed.  gkm_get_state
+ * copies the state either way. */
 int gkm_step(gkm_t* h, double dt, int nsteps);
 
 /* dt = cfl*dx / max_interior(|u| + sqrt(gamma p/rho)), FP64 device
@@ -249,7 +260,8 @@ int gkm_step_print_stats(gkm_t* h, double dt, int nsteps, gkm_print_stats_t* out
 int gkm_format_print_stats(const gkm_print_stats_t* s, double t_over_t0, double gamma, double p0, double T0,
                            double Mt, double den0, char* buf, int len);
 
-/* Wait for all work queued on the handle's stream. */
+/* Wait for all work queued on the handle's stream; GKM_E_NONPHYSICAL if a
+ * step left a non-physical state (see gkm_step). */
 int gkm_sync(gkm_t* h);
 
 /* Device time of the last gkm_step call, from HIP events recorded on the
