@@ -37,6 +37,7 @@ EXPORTS = (
     "gkm_print_stats", "gkm_print_stats_box", "gkm_step_print_stats", "gkm_format_print_stats", "gkm_gks_points",
     "gkm_last_sweep_timing", "gkm_get_grid", "gkm_get_params_box", "gkm_set_params_box",
     "gkm_set_dt_batch", "gkm_group_create", "gkm_group_step", "gkm_group_sync", "gkm_group_destroy",
+    "gkm_flux_points_noalloc",
     # include/gkm_ic.h
     "gkm_dhit_velocity", "gkm_init_dhit_box", "gkm_init_dhit",
     # include/gkm_io.h
@@ -146,6 +147,7 @@ def load() -> C.CDLL:
     L.gkm_last_timing.argtypes = [vp, DP, C.POINTER(C.c_int)]
     L.gkm_last_sweep_timing.argtypes = [vp, DP]
     L.gkm_flux_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
+    L.gkm_flux_points_noalloc.argtypes = [C.POINTER(GkmParams), C.c_int, DP, C.c_double, DP]
     L.gkm_weno_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
     L.gkm_gks_points.argtypes = [C.POINTER(GkmParams), C.c_int, DP, DP]
     L.gkm_create_batch.argtypes = [C.POINTER(GkmGrid), C.POINTER(GkmParams), C.c_int, C.POINTER(vp)]

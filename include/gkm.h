@@ -300,6 +300,16 @@ int gkm_weno_points(const gkm_params* p, int n, const double* in, double* out);
  *   in[n*22] = W0[5] DWx[5] DWy[5] DWz[5] mu dt (rotated slot order);  F[n*5] */
 int gkm_gks_points(const gkm_params* p, int n, const double* in, double* F);
 
+/* gkm_flux_points with no device allocation per call.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Records
+ * and fluxes are staged through a static device array of the library (3.7 MB
+ * per GPU, held until the process ends) instead of per-call buffers, on the
+ * null stream; the results equal gkm_flux_points' bit for bit.  n may be at
+ * most 8192 (GKM_E_ARG beyond).  Calls on one GPU share the array and are
+ * serialised by a lock in the library. */
+int gkm_flux_points_noalloc(const gkm_params* p, int n, const double* in, double dx, double* F);
+
 const char* gkm_last_error(void);
 
 /* Library version string, e.g. "gkm-mi355x 0.1 gfx950". */
