@@ -1,14 +1,17 @@
-"""The FP32 flux's erfc: erfc(x) = erfcx(min(|x|, 12)) exp(-x^2) with erfcx a
-degree 4 / degree 5 rational (gkm-dns_amd/csrc/gkm_erfcx.h, written by
-gen_erfcx.py from mpmath values), erfc(-|x|) = 2 - erfc(|x|).
+"""The flux's erfc: erfc(x) = erfcx(min(|x|, cap)) exp(-x^2) with erfcx a
+rational (gkm-dns_amd/csrc/gkm_erfcx.h, written by gen_erfcx.py from mpmath
+values), erfc(-|x|) = 2 - erfc(|x|): degree 4 / 5 on [0, 12] in the FP32 flux,
+degree 9 / 10 on [0, 27] in the FP64 flux.
 
 Pinned against scipy.special.erfc / erfcx (an independent implementation):
-the rational with its float coefficients, evaluated in FP64, is within 5e-8
+FP32: the rational with its float coefficients, evaluated in FP64, is within 5e-8
 relative of erfcx on [0, 12] (the fit itself: 8e-9; rounding the coefficients
 to float adds the rest) and decays like 1/(x sqrt(pi)) as erfcx does; the FP32
 evaluation the kernel performs (Horner with FMA, hardware reciprocal) is
 within 5e-7 and stays within the error of the FP32 exponential it multiplies
-by.
+by.  FP64: the rational evaluated as the kernel does (Horner with FMA, one
+division) is within 2e-15 of erfcx on [0, 27] and erfc formed from it within
+3e-15 where exp(-x^2) carries no argument rounding of its own.
 """
 import math
 import re
@@ -20,16 +23,16 @@ from scipy.special import erfc, erfcx
 HDR = Path(__file__).resolve().parents[1] / "gkm-dns_amd" / "csrc" / "gkm_erfcx.h"
 
 
-def _table():
+def _table(sfx=""):
     s = HDR.read_text()
-    cap = float(re.search(r"kErfcxCap = ([0-9.eE+-]+);", s).group(1))
+    cap = float(re.search(r"kErfcxCap" + sfx + r" = ([0-9.eE+-]+);", s).group(1))
 
     def coefs(name):
         body = re.search(name + r"\[\d+\] = \{(.*?)\};", s, re.S).group(1)
         return [float(t.strip().rstrip("f")) for t in body.split(",") if t.strip()]
-    p, q = coefs("kErfcxP"), coefs("kErfcxQ")
-    assert len(p) == int(re.search(r"kErfcxDegP = (\d+);", s).group(1)) + 1
-    assert len(q) == int(re.search(r"kErfcxDegQ = (\d+);", s).group(1)) + 1
+    p, q = coefs("kErfcxP" + sfx), coefs("kErfcxQ" + sfx)
+    assert len(p) == int(re.search(r"kErfcxDegP" + sfx + r" = (\d+);", s).group(1)) + 1
+    assert len(q) == int(re.search(r"kErfcxDegQ" + sfx + r" = (\d+);", s).group(1)) + 1
     return cap, p, q
 
 
@@ -81,3 +84,39 @@ def test_float_erfc_as_the_kernel_forms_it():
     with np.errstate(over="ignore"):
         eb = np.exp(-(big * big)).astype(np.float32)
     assert np.all((_erfcx(big, cap, p, q, np.float32) * eb) == 0)
+
+
+def _erfcx64(a, cap, p, q):
+    """The FP64 kernel's evaluation: Horner with fused multiply-add, P / Q."""
+    x = np.minimum(np.asarray(a, np.float64), cap)
+    P = np.full_like(x, p[-1])
+    for c in p[-2::-1]:
+        P = P * x + c
+    Q = np.full_like(x, q[-1])
+    for c in q[-2::-1]:
+        Q = Q * x + c
+    return P / Q
+
+
+def test_fp64_rational():
+    cap, p, q = _table("64")
+    assert cap == 27.0 and len(p) == 10 and len(q) == 11
+    assert all(c > 0 for c in p + q)
+    a = np.concatenate([np.linspace(0.0, 1.0, 100001), np.linspace(1.0, cap, 400001)])
+    rel = np.abs(_erfcx64(a, cap, p, q) / erfcx(a) - 1)
+    assert rel.max() < 2e-15, rel.max()
+    assert abs(p[-1] / q[-1] * math.sqrt(math.pi) - 1) < 1e-6
+
+
+def test_fp64_erfc_as_the_kernel_forms_it():
+    cap, p, q = _table("64")
+    # values whose square is exact, so exp(-x^2) adds only its own rounding
+    x = np.arange(-2 ** 14, 2 ** 14 + 1) / 2 ** 9          # [-32, 32] in steps of 2^-9
+    a = np.abs(x)
+    e = np.exp(-(a * a))
+    r = _erfcx64(a, cap, p, q) * e
+    got = np.where(x >= 0, r, 2.0 - r)
+    ref = erfc(x)
+    nz = ref > 1e-300
+    assert np.max(np.abs(got[nz] / ref[nz] - 1)) < 3e-15
+    assert np.all(got[~nz] < 1e-290) and np.all(np.isfinite(got))
