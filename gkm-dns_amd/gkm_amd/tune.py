@@ -4,12 +4,16 @@ For a box size (and precision / reconstruction), every candidate pair
 (flux threads per block in {64, 128, 256}, WENO interfaces per thread of the
 strided sweeps in {4, 8, 16}) is timed on a batch of boxes of that size with
 device events around the step, after a warm-up step; the fastest pair is kept.
-Results are cached as JSON, keyed by "n/precision/recon/nbox", so later runs
-read them instead of timing again.  The arithmetic of every candidate is the
-same: the choice changes speed, never results.
+Results are cached as JSON, keyed by the built library's identity (its
+version string and a hash of libgkm.so, so a rebuilt kernel is timed again)
+and "n/precision/recon/nbox", so later runs read them instead of timing again.
+The arithmetic of every candidate is the same: the choice changes speed, never
+results.
 """
 from __future__ import annotations
 
+import functools
+import hashlib
 import itertools
 import json
 import os
@@ -18,6 +22,7 @@ from pathlib import Path
 
 import numpy as np
 
+from . import _abi
 from .ic import dhit
 from .solver import Batch
 
@@ -32,6 +37,18 @@ def cache_path() -> Path:
     if env:
         return Path(env)
     return Path.home() / ".cache" / "gkm_amd" / "tuning.json"
+
+
+@functools.lru_cache(maxsize=None)
+def library_id() -> str:
+    """gkm_version() and the first 16 hex digits of the SHA-1 of libgkm.so."""
+    L = _abi.load()
+    digest = hashlib.sha1(Path(_abi.LIB_PATH).read_bytes()).hexdigest()[:16]
+    return f"{L.gkm_version().decode()}@{digest}"
+
+
+def _key(n: int, precision: str, recon: str, nbox: int) -> str:
+    return f"{library_id()}/{n}/{precision}/{recon}/{nbox}"
 
 
 def _load(path: Path) -> dict:
@@ -51,7 +68,7 @@ def _store(path: Path, data: dict) -> None:
 
 def tune_size(n: int, precision: str = "mixed", recon: str = "js", nbox: int = 4, steps: int = 3,
               use_cache: bool = True, verbose: bool = False) -> tuple[int, int]:
-    key = f"{n}/{precision}/{recon}/{nbox}"
+    key = _key(n, precision, recon, nbox)
     path = cache_path()
     cache = _load(path) if use_cache else {}
     if key in cache:
@@ -86,7 +103,7 @@ def speedup_report(sizes, precision: str = "mixed", recon: str = "js", nbox: int
     cache = _load(cache_path())
     rep = {}
     for n in sizes:
-        e = cache.get(f"{n}/{precision}/{recon}/{nbox}")
+        e = cache.get(_key(n, precision, recon, nbox))
         if e:
             best = min(e["ms_per_step"].values())
             rep[int(n)] = {"best": e["best"], "best_ms": best, "default_ms": e["default_ms"],

@@ -176,3 +176,16 @@ def test_null_handles_are_rejected():
     assert L.gkm_batch_size(None, C.byref(n)) == A.GKM_E_ARG
     assert L.gkm_set_graph(None, 1) == A.GKM_E_ARG
     assert L.gkm_set_timing(None, 1) == A.GKM_E_ARG
+
+
+def test_tuning_cache_key_names_the_built_library():
+    """The autotuner's cache key carries the library version and a hash of
+    libgkm.so, so launch shapes tuned for one build are not reused by another."""
+    import hashlib
+
+    import gkm_amd._abi as A
+    from gkm_amd import tune
+    lid = tune.library_id()
+    assert lid.startswith(A.load().gkm_version().decode() + "@")
+    assert lid.endswith(hashlib.sha1(A.LIB_PATH.read_bytes()).hexdigest()[:16])
+    assert tune._key(64, "mixed", "js", 4) == f"{lid}/64/mixed/js/4"
