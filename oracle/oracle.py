@@ -16,6 +16,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "_build" / "liboracle.so"
 REF_SO = HERE / "_ref" / "libref_flux.so"
+OPCOUNT_SO = HERE / "_build" / "libopcount.so"
 
 DP = C.POINTER(C.c_double)
 
@@ -237,6 +238,43 @@ def print_stats(grid: Grid, params: Params, W) -> dict:
     s = PrintStats()
     lib().orc_print_stats(C.byref(grid), C.byref(params), _ptrs(W), C.byref(s))
     return s.as_dict()
+
+
+OPCOUNT_FIELDS = ("add", "mul", "div", "sqrt", "exp", "erfc", "pow")
+
+
+def opcount(params: Params) -> dict:
+    """Operation counts of the oracle (oracle/opcount.cpp) for one interface's
+    WENO, one interface's flux and one cell's update:
+    {"weno": {...}, "flux": {...}, "update": {...}} with add (incl. sub), mul,
+    div, sqrt (the arithmetic flops) and exp, erfc, pow (transcendentals)."""
+    if not OPCOUNT_SO.exists():
+        raise RuntimeError(f"op counter not built: {OPCOUNT_SO} (run `make -C oracle`)")
+    L = C.CDLL(str(OPCOUNT_SO))
+    L.orc_opcount.argtypes = [C.POINTER(Params), C.POINTER(C.c_ulonglong)]
+    out = (C.c_ulonglong * 21)()
+    L.orc_opcount(C.byref(params), out)
+    return {part: dict(zip(OPCOUNT_FIELDS, out[7 * k:7 * k + 7]))
+            for k, part in enumerate(("weno", "flux", "update"))}
+
+
+def flops_per_cell_update(params: Params, n: int | None = None) -> dict:
+    """Algorithmic work per cell-update of the scheme as the oracle states it:
+    three sweeps of (WENO per interface + flux per interface) plus the update.
+    With n, the interface counts of an n^3 box (WENO on the extended set
+    (n+1)(n+2)^2, flux on the core set (n+1) n^2, per sweep); without, the
+    large-box limit of one of each per cell and sweep."""
+    c = opcount(params)
+    arith = {k: c[k]["add"] + c[k]["mul"] + c[k]["div"] + c[k]["sqrt"] for k in c}
+    trans = {k: c[k]["exp"] + c[k]["erfc"] + c[k]["pow"] for k in c}
+    if n is None:
+        we = fl = 1.0
+    else:
+        we = (n + 1) * (n + 2) ** 2 / n ** 3
+        fl = (n + 1) / n
+    return {"flops": 3 * (we * arith["weno"] + fl * arith["flux"]) + arith["update"],
+            "transcendentals": 3 * (we * trans["weno"] + fl * trans["flux"]) + trans["update"],
+            "per_part": c}
 
 
 def build(ref: bool = True) -> None:
