@@ -99,6 +99,7 @@ void free_ctx(Ctx* h) {
     if (h->spl_grad) (void)hipFree(h->spl_grad);
     if (h->spl_inv) (void)hipFree(h->spl_inv);
     if (h->spl_wt) (void)hipFree(h->spl_wt);
+    if (h->spl_part) (void)hipFree(h->spl_part);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->ps_host) (void)hipHostFree(h->ps_host);
     if (h->h_bad) (void)hipHostFree(h->h_bad);
@@ -872,6 +873,7 @@ static int ensure_spline_buffers(Ctx* h) {
     HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc((void**)&h->spl_wt, sizeof(double) * wt.size()));
     HIP_TRY(hipMemcpy(h->spl_wt, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc((void**)&h->spl_part, sizeof(double) * gkm::print_stats_part_doubles(g)));
     return GKM_OK;
 }
 

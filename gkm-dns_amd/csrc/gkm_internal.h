@@ -85,6 +85,7 @@ struct Ctx {
     double* spl_grad = nullptr;
     double* spl_inv = nullptr;
     double* spl_wt = nullptr;  // backward-sum weights of the y (P0 rows) and x (P1 rows) lines, 32 per row
+    double* spl_part = nullptr;  // partial-sum rows of the fused PrintStats passes (print_stats_part_doubles)
     // gkm_step_print_stats: a second stream (default priority) for the
     // statistics, its events and the pinned host rows of the sums
     hipStream_t stream2 = nullptr;
@@ -131,5 +132,7 @@ constexpr int kPStat = 15;  // PrintStats sums
 constexpr int kSplRing = 32;  // terms of the backward sums of the x and y line slopes
 // spline slopes into h->spl_grad (allocated by the caller), then the sums
 hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host_out /*kPStat + kStatB*/);
+// doubles of h->spl_part the fused PrintStats passes need for this grid
+size_t print_stats_part_doubles(const Geo& g);
 
 }  // namespace gkm

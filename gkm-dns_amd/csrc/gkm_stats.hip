@@ -60,6 +60,7 @@ __device__ __forceinline__ void vel(const St& st, long c, double u[3]) {
 
 struct StatPar {
     double gam, R, mu0, T0, visc_exp, dx;
+    double iR, iT0;  // 1/R, 1/T0
     int visc_law;
 };
 
@@ -202,6 +203,8 @@ static StatPar stat_par(const Ctx* h, int box) {
     p.T0 = q.T0;
     p.visc_exp = q.visc_exp;
     p.dx = h->grid.dx;
+    p.iR = 1.0 / q.R;
+    p.iT0 = 1.0 / q.T0;
     p.visc_law = q.visc_law;
     return p;
 }
@@ -274,13 +277,17 @@ hipError_t launch_stats_b(Ctx* h, int box, const double m[4], double* host_out) 
 // inv_i, backward D_i = y_i - inv_i D_{i+1}.  Only interior nodes' slopes are
 // used.
 //
-// Each axis is one pass that writes its slopes for the interior nodes to its
-// three planes ((3 A + comp) * ncell; A = 0 y, 1 x, 2 z); k_pstats then forms
-// the sums from the nine planes and the state.  x and y lines are streamed,
-// a lane per line and component (k_spl_stream).  z lines, contiguous, are
-// solved whole in LDS when they have up to kSplMaxM nodes (boxes up to 335
-// cells a side, k_spl_lines); longer ones go in chunks through the gradient
-// planes (k_spl_chunked).
+// One pass per axis.  z lines, contiguous, are solved whole in LDS when they
+// have up to kSplMaxM nodes (boxes up to 335 cells a side); x and y lines are
+// streamed, a lane per line and component (k_spl_stream).  Each slope is added
+// to the sums where it is made (k_spl_lines_sums for z, which also takes the
+// sums that need only the state, then the x and y passes): only mu(T) and the
+// three slopes of the S_ij cross terms that a later pass pairs with its own
+// (dudz, dvdz, dvdx) cross HBM as planes, and each block writes one row of
+// partial sums that k_pstats_final reduces.  Longer z lines go in chunks
+// through the gradient planes (k_spl_chunked); then every axis writes its
+// three planes ((3 A + comp) * ncell; A = 0 y, 1 x, 2 z) and k_pstats forms
+// the sums from the nine planes and the state.
 // ---------------------------------------------------------------------------
 constexpr int kSpLines = 64, kSpThreads = 192, kSpChunk = 16;
 
@@ -618,17 +625,60 @@ template <> struct SplRaw<StateD> {
     __device__ __forceinline__ double f() const { return wq / w0; }
 };
 
-template <class St, int A>
+// The slope planes the fused passes exchange (planes of h->spl_grad): mu(T)
+// and dudz, dvdz from the z pass, dvdx from the x pass.
+enum : int { kPlMu, kPlDudz, kPlDvdz, kPlDvdx };
+
+// What one slope d = d(u_q)/d(axis A) adds to the PrintStats sums (slot
+// layout of pstat_add), given mu and the partner slope of its S_ij cross term:
+//   mu sum_ij (du_i/dx_j)^2   each slope once, mu d^2 (slot 1)
+//   S_ij S_ij                 (du/dx)^2 + (dv/dy)^2 + (dw/dz)^2 whole, the
+//                             off-diagonal squares halved, and the three
+//                             cross products dudy dvdx, dudz dwdx, dwdy dvdz
+//                             added by the pass that holds the second factor
+//                             (y pass: u with dvdx, w with dvdz; x pass: w
+//                             with dudz) (slot 2)
+//   (du/dx)^n, (dv/dy)^n, (dw/dz)^n, n = 2, 3, 4   (slots 3..11)
+struct SlopeSums {
+    double mu_g2 = 0.0, sij = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0;
+    __device__ __forceinline__ void add(double d, double mu, bool diag, double partner) {
+        const double d2v = d * d;
+        mu_g2 += mu * d2v;
+        sij += diag ? d2v : 0.5 * d2v + d * partner;
+        if (diag) {
+            d2 += d2v;
+            d3 += d2v * d;
+            d4 += d2v * d2v;
+        }
+    }
+    // into the kPSum slots; diag_slot = 3, 6 or 9 for the diagonal slope's moments
+    __device__ __forceinline__ void put(double (&acc)[kPSum], int diag_slot) const {
+        acc[1] += mu_g2;
+        acc[2] += sij;
+        if (diag_slot == 3) { acc[3] += d2; acc[4] += d3; acc[5] += d4; }
+        if (diag_slot == 6) { acc[6] += d2; acc[7] += d3; acc[8] += d4; }
+        if (diag_slot == 9) { acc[9] += d2; acc[10] += d3; acc[11] += d4; }
+    }
+};
+
+// SUMS = false: the slopes go to the axis's three planes for k_pstats.
+// SUMS = true (the fused PrintStats): each slope is added to the sums where it
+// is made, and only the planes a later pass needs are written (x pass:
+// dvdx); each block writes one row of kPSum partial sums to `part`.
+template <class St, int A, bool SUMS>
 __global__ void __launch_bounds__(kSplStreamThreads) k_spl_stream(Geo g, St st, const double* __restrict__ inv,
                                                    const double* __restrict__ wt, double r3h,
-                                                   double* __restrict__ grad) {
+                                                   double* __restrict__ grad, double* __restrict__ part) {
     static_assert(A != 2, "z lines go through k_spl_lines");
     static_assert(kSplRing % kSplAhead == 0, "queue slots fixed within the unrolled ring");
     const SplAxis<A> ax(g);
     const unsigned nl = ax.n_in * ax.n_out;
     const unsigned t = blockIdx.x * 64u + (threadIdx.x & 63u);
     const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // this wavefront's component
-    if (t >= nl) return;
+    SlopeSums sums;
+    if (!SUMS && t >= nl) return;
+    // with SUMS every thread reaches the block reduction below
+    if (t < nl) {
     const int m = ax.m;
     const unsigned b = ax.base(g, t), s = ax.stride;
     double* G = grad + (size_t)(3 * A + q) * g.ncell;
@@ -666,31 +716,147 @@ __global__ void __launch_bounds__(kSplStreamThreads) k_spl_stream(Geo g, St st, 
                 double d = 0.0;
 #pragma unroll
                 for (int j = kSplRing - 1; j >= 0; j--) d = fma(w[j], ring[(u + 1 + j) % kSplRing], d);
-                put(G, b + (unsigned)i * s, d);
+                const unsigned c = b + (unsigned)i * s;
+                if constexpr (!SUMS) {
+                    put(G, c, d);
+                } else {
+                    const double mu = ld(grad + (size_t)kPlMu * g.ncell, c);
+                    if constexpr (A == 1) {  // x: d = d(u_q)/dx
+                        if (q == 1) put(grad + (size_t)kPlDvdx * g.ncell, c, d);
+                        sums.add(d, mu, q == 0, q == 2 ? ld(grad + (size_t)kPlDudz * g.ncell, c) : 0.0);
+                    } else {  // y: d = d(u_q)/dy
+                        const double pt = q == 0   ? ld(grad + (size_t)kPlDvdx * g.ncell, c)
+                                          : q == 2 ? ld(grad + (size_t)kPlDvdz * g.ncell, c)
+                                                   : 0.0;
+                        sums.add(d, mu, q == 1, pt);
+                    }
+                }
             }
         }
     }
+    }
+    if constexpr (SUMS) {
+        double acc[kPSum];
+#pragma unroll
+        for (int k = 0; k < kPSum; k++) acc[k] = 0.0;
+        sums.put(acc, A == 1 ? (q == 0 ? 3 : -1) : (q == 1 ? 6 : -1));
+        block_reduce_store<kPSum, 0u, kSplStreamThreads / 64>(acc, part);
+    }
 }
 
-// lines of axis A: interior slopes into planes 3 A .. 3 A + 2
-// (LDS limits a CU to three blocks, at most three wavefronts per SIMD; saying so
-// lets the compiler keep a whole batch of loads in flight instead of budgeting
-// registers for eight wavefronts)
-template <class St, int A>
-__global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_eu(1, 3)))
-    k_spl_lines(Geo g, St st, const double* __restrict__ inv,
-                                                           double r3h, double* __restrict__ grad) {
-    extern __shared__ double spl_tile[];
-    const SplAxis<A> ax(g);
-    const unsigned nl = ax.n_in * ax.n_out;
-    const unsigned t0 = xcd_block() * kSplL;
-    if (t0 >= nl) return;  // block-uniform
-    spl_group_solve(g, st, ax, t0, nl, inv, r3h, spl_tile);
-    const int m = ax.m;
-    spl_store(g, ax, t0, nl, 3, m - 3, [&](int l, int n, unsigned c) {
+// The fused z pass: the z lines' slopes as k_spl_lines makes them, then, at
+// every interior node of the group, the sums that need the state (rho|u|^2,
+// p, T, 1/rho and their shifted sums) and the z slopes' terms; mu(T), dudz
+// and dvdz go to their planes for the x and y passes.  One row of kPSum
+// partial sums per block.
+// The five state words of a cell, loaded before they are combined (the
+// mixed state's hi and lo floats, or the FP64 values).
+template <class St> struct Raw5;
+template <> struct Raw5<StateM> {
+    float h[5], l[5];
+    __device__ __forceinline__ void load(const StateM& st, unsigned c) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) put(grad + (size_t)(3 * A + q) * g.ncell, c, spl_tile[(q * m + n) * kSplL + l]);
-    });
+        for (int q = 0; q < 5; q++) {
+            h[q] = ld(st.hi[q], c);
+            l[q] = ld(st.lo[q], c);
+        }
+    }
+    __device__ __forceinline__ double get(int q) const { return (double)h[q] + (double)l[q]; }
+};
+template <> struct Raw5<StateD> {
+    double w[5];
+    __device__ __forceinline__ void load(const StateD& st, unsigned c) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) w[q] = ld(st.w[q], c);
+    }
+    __device__ __forceinline__ double get(int q) const { return w[q]; }
+};
+
+// rho, |u|^2, p, T and 1/rho of a cell with one reciprocal of rho (the
+// fused pass's form of cell_pt), and mu(T) with the power law as
+// exp2(e log2(T/T0)) (the device library's pow is ~220 instructions)
+template <class R>
+__device__ __forceinline__ void cell_fast(const R& r, const StatPar& p, double& rho, double& q2, double& P,
+                                          double& T, double& iv, double& mu) {
+    rho = r.get(0);
+    iv = 1.0 / rho;
+    const double u = r.get(1) * iv, v = r.get(2) * iv, w = r.get(3) * iv;
+    q2 = u * u + v * v + w * w;
+    P = (rho * (p.gam - 1)) * (r.get(4) * iv - 0.5 * q2);
+    T = P * p.iR * iv;
+    const double x = T * p.iT0;
+    mu = p.visc_law == 1 ? p.mu0 * exp2(p.visc_exp * log2(x))
+                         : p.mu0 * x * sqrt(x) * (p.T0 + 110.4) / (T + 110.4);
+}
+
+template <class St>
+__global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_eu(1, 3)))
+    k_spl_lines_sums(Geo g, St st, const double* __restrict__ inv, double r3h, StatPar p, int gpb,
+                     double* __restrict__ grad, double* __restrict__ part) {
+    extern __shared__ double spl_tile[];
+    const SplAxis<2> ax(g);
+    const unsigned nl = ax.n_in * ax.n_out;
+    double acc[kPSum];
+#pragma unroll
+    for (int k = 0; k < kPSum; k++) acc[k] = 0.0;
+    double K[3];  // the deviation sums' shift: p, T, 1/rho of the first interior cell
+    {
+        double rho, q2;
+        cell_pt(st, 3 * g.s0 + 3 * g.s1 + 3, p, rho, q2, K[0], K[1]);
+        K[2] = 1.0 / rho;
+    }
+    SlopeSums sums;
+    // node n of line l at cell c: the state sums and the z slopes' terms;
+    // mu(T), dudz and dvdz to their planes
+    auto body_node = [&](const Raw5<St>& r, unsigned c, int l, int n, int m) {
+        double rho, q2, P, T, iv, mu;
+        cell_fast(r, p, rho, q2, P, T, iv, mu);
+        const double dz[3] = {spl_tile[(0 * m + n) * kSplL + l], spl_tile[(1 * m + n) * kSplL + l],
+                              spl_tile[(2 * m + n) * kSplL + l]};
+        put(grad + (size_t)kPlMu * g.ncell, c, mu);
+        put(grad + (size_t)kPlDudz * g.ncell, c, dz[0]);
+        put(grad + (size_t)kPlDvdz * g.ncell, c, dz[1]);
+        sums.add(dz[0], mu, false, 0.0);  // dudz: its cross term dudz dwdx is added by the x pass
+        sums.add(dz[1], mu, false, 0.0);  // dvdz: dwdy dvdz by the y pass
+        sums.add(dz[2], mu, true, 0.0);
+        acc[0] += rho * q2;
+        acc[12] += P; acc[13] += T; acc[14] += iv;
+        const double dp = P - K[0], dT = T - K[1], dv = iv - K[2];
+        acc[15] += dp; acc[16] += dT; acc[17] += dv;
+        acc[18] += dp * dp; acc[19] += dT * dT; acc[20] += dv * dv;
+    };
+    // gpb consecutive groups per block, the blocks of an XCD on neighbouring groups
+    for (int grp = 0; grp < gpb; grp++) {
+        const unsigned t0 = (xcd_block() * (unsigned)gpb + (unsigned)grp) * kSplL;
+        if (t0 >= nl) break;  // block-uniform
+        if (grp > 0) __syncthreads();  // the previous group's slopes have been read
+        spl_group_solve(g, st, ax, t0, nl, inv, r3h, spl_tile);
+        const int m = ax.m;
+        // the group's interior nodes, element e = (line e / ni, node 3 + e % ni),
+        // consecutive threads on consecutive nodes; the state words of kSplBatch
+        // elements are loaded before any is combined
+        const int ni = m - 6;
+        const int ne = (int)min((unsigned)kSplL, nl - t0) * ni;
+        constexpr int kSplBatch = 4;
+        for (int e0 = threadIdx.x; e0 < ne; e0 += kSplBatch * kSplThreads) {
+            Raw5<St> raw[kSplBatch];
+            unsigned cc[kSplBatch];
+#pragma unroll
+            for (int k = 0; k < kSplBatch; k++) {
+                const int e = min(e0 + k * kSplThreads, ne - 1);  // past the end: repeat the last, unused
+                const int l = e / ni, n = 3 + e % ni;
+                cc[k] = ax.base(g, t0 + (unsigned)l) + (unsigned)n;
+                raw[k].load(st, cc[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kSplBatch; k++) {
+                const int e = e0 + k * kSplThreads;
+                if (e < ne) body_node(raw[k], cc[k], e / ni, 3 + e % ni, m);
+            }
+        }
+    }
+    sums.put(acc, 9);
+    block_reduce_store<kPSum, 0u, kSplThreads / 64>(acc, part);
 }
 
 // The sums over the interior from the nine slope planes and the state,
@@ -744,28 +910,58 @@ __global__ void __launch_bounds__(kRedThreads) k_pstats_final(int nb, const doub
 // The slopes of the three axes into their planes (whole lines in LDS where
 // they fit, m <= kSplMaxM, else in chunks), then the sums; returns the number
 // of partial rows.
+// Rows of partial sums the fused passes write: one per kSplGroups z-line
+// groups, one per 64 x lines, one per 64 y lines.
+constexpr int kSplGroups = 4;
+static void fused_rows(const Geo& g, unsigned rows[3]) {
+    const unsigned groups = ((unsigned)g.ny * g.nx + kSplL - 1) / kSplL;
+    rows[0] = (groups + kSplGroups - 1) / kSplGroups;
+    rows[1] = ((unsigned)g.ny * g.nz + 63) / 64;
+    rows[2] = ((unsigned)g.nx * g.nz + 63) / 64;
+}
+
+size_t print_stats_part_doubles(const Geo& g) {
+    unsigned r[3];
+    fused_rows(g, r);
+    return (size_t)(r[0] + r[1] + r[2]) * kPSum;
+}
+
+// Returns the number of partial rows in *part_out.
 template <class St>
-static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStream_t s) {
+static int launch_print_stats_st(Ctx* h, const St& st, const StatPar& p, hipStream_t s, const double** part_out) {
     const Geo& g = h->geo;
     const double r3h = 3.0 / h->grid.dx;
     const double* inv[3] = {h->spl_inv, h->spl_inv + g.P0, h->spl_inv + g.P0 + g.P1};
     const int m[3] = {g.P0, g.P1, g.P2};
     const unsigned nl[3] = {(unsigned)g.nx * g.nz, (unsigned)g.ny * g.nz, (unsigned)g.ny * g.nx};
     auto tile_bytes = [](int mm) { return sizeof(double) * 3 * (size_t)mm * kSplL; };
-    if (m[2] <= kSplMaxM)
-        hipLaunchKernelGGL((k_spl_lines<St, 2>), dim3((nl[2] + kSplL - 1) / kSplL), dim3(kSplThreads),
-                           tile_bytes(m[2]), s, g, st, inv[2], r3h, h->spl_grad);
-    else
-        hipLaunchKernelGGL((k_spl_chunked<St>), dim3((nl[2] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0, s, g,
-                           st, inv[2], r3h, h->spl_grad);
     const unsigned sb[2] = {(nl[0] + 63) / 64, (nl[1] + 63) / 64};
     const dim3 sth(kSplStreamThreads);
-    hipLaunchKernelGGL((k_spl_stream<St, 1>), dim3(sb[1]), sth, 0, s, g, st, inv[1],
-                       h->spl_wt + (size_t)g.P0 * kSplRing, r3h, h->spl_grad);
-    hipLaunchKernelGGL((k_spl_stream<St, 0>), dim3(sb[0]), sth, 0, s, g, st, inv[0], h->spl_wt, r3h,
-                       h->spl_grad);
+    const double* wt_x = h->spl_wt + (size_t)g.P0 * kSplRing;
+    if (m[2] <= kSplMaxM) {
+        // fused: the sums are taken where the slopes are made (z, then x, then y)
+        unsigned rows[3];
+        fused_rows(g, rows);
+        double* part = h->spl_part;
+        hipLaunchKernelGGL(k_spl_lines_sums<St>, dim3(rows[0]), dim3(kSplThreads), tile_bytes(m[2]), s, g, st,
+                           inv[2], r3h, p, kSplGroups, h->spl_grad, part);
+        hipLaunchKernelGGL((k_spl_stream<St, 1, true>), dim3(sb[1]), sth, 0, s, g, st, inv[1], wt_x, r3h,
+                           h->spl_grad, part + (size_t)rows[0] * kPSum);
+        hipLaunchKernelGGL((k_spl_stream<St, 0, true>), dim3(sb[0]), sth, 0, s, g, st, inv[0], h->spl_wt, r3h,
+                           h->spl_grad, part + (size_t)(rows[0] + rows[1]) * kPSum);
+        *part_out = part;
+        return (int)(rows[0] + rows[1] + rows[2]);
+    }
+    // z lines too long for LDS: the nine slope planes, then the sums from them
+    hipLaunchKernelGGL((k_spl_chunked<St>), dim3((nl[2] + kSpLines - 1) / kSpLines), dim3(kSpThreads), 0, s, g, st,
+                       inv[2], r3h, h->spl_grad);
+    hipLaunchKernelGGL((k_spl_stream<St, 1, false>), dim3(sb[1]), sth, 0, s, g, st, inv[1], wt_x, r3h,
+                       h->spl_grad, nullptr);
+    hipLaunchKernelGGL((k_spl_stream<St, 0, false>), dim3(sb[0]), sth, 0, s, g, st, inv[0], h->spl_wt, r3h,
+                       h->spl_grad, nullptr);
     hipLaunchKernelGGL(k_pstats<St>, dim3(h->red_blocks), dim3(kRedThreads), 0, s, g, st, h->spl_grad, p,
                        make_fastdiv((unsigned)g.nz), make_fastdiv((unsigned)g.nx), h->red_part);
+    *part_out = h->red_part;
     return h->red_blocks;
 }
 
@@ -777,19 +973,20 @@ hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host
     const StatPar p = stat_par(h, box);
     const size_t off = (size_t)box * h->ncell;
     int nb;
+    const double* part = nullptr;
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q] + off;
-        nb = launch_print_stats_st(h, st, p, s);
+        nb = launch_print_stats_st(h, st, p, s, &part);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q] + off;
             st.lo[q] = h->Wlo[q] + off;
         }
-        nb = launch_print_stats_st(h, st, p, s);
+        nb = launch_print_stats_st(h, st, p, s, &part);
     }
-    hipLaunchKernelGGL(k_pstats_final, dim3(1), dim3(kRedThreads), 0, s, nb, h->red_part,
+    hipLaunchKernelGGL(k_pstats_final, dim3(1), dim3(kRedThreads), 0, s, nb, part,
                        (double)h->grid.nx * h->grid.ny * h->grid.nz, h->red_out);
     return hipMemcpyAsync(host_out, h->red_out, sizeof(double) * (kPStat + kStatB), hipMemcpyDeviceToHost, s);
 }
