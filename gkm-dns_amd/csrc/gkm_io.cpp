@@ -7,8 +7,12 @@
 #include <string>
 #include <vector>
 
+#include "../../include/gkm.h"
 #include "../../include/gkm_io.h"
-#include "gkm_internal.h"
+
+namespace gkm {
+int report_error(int code, const char* msg);
+}
 
 namespace {
 
