@@ -293,10 +293,15 @@ def bench_ensemble(args):
     ek = np.array([abs(a["ek"] - b["ek"]) / abs(b["ek"]) for a, b in zip(sm, sd)])
     mt = np.array([abs(a["mt"] - b["mt"]) / abs(b["mt"]) for a, b in zip(sm, sd)])
     cells = em.cells
-    # the same ensemble advanced by one graph launch per step (gkm_group_step)
+    # the same ensemble advanced by one graph launch per step (gkm_group_step),
+    # and by one persistent cooperative launch for all the timed steps
+    # (gkm_group_step_persistent)
     em.use_group(True)
     em.step(2)
     sec_g = em.run_timed(args.ensemble_steps)
+    em.use_group(True, persistent=True)
+    em.step(2)
+    sec_p = em.run_timed(args.ensemble_steps)
     em.close()
     ed.close()
     print(f"[bench] ensemble: {1e3 * sec / args.ensemble_steps:.2f} ms/step mixed, "
@@ -306,6 +311,7 @@ def bench_ensemble(args):
             "ms_per_step": 1e3 * sec / args.ensemble_steps,
             "fp64_value": cells * args.ensemble_steps / sec_d,
             "one_graph_launch_value": cells * args.ensemble_steps / sec_g,
+            "one_persistent_launch_value": cells * args.ensemble_steps / sec_p,
             "ek_rel_err_max": float(ek.max()), "ek_rel_err_median": float(np.median(ek)),
             "mt_rel_err_max": float(mt.max()),
             "tuning": {str(n): list(v) for n, v in tuning.items()}, "tune_seconds": t_tune,

@@ -514,6 +514,13 @@ struct gkm_group {
     };
     std::vector<Cap> caps;
     std::vector<unsigned> gens;  // the handles' tuning generations the captures were made with
+    // gkm_group_step_persistent: class tables in device memory, one per
+    // combination of the handles' current buffers (keyed like the captures)
+    struct PTab {
+        std::vector<int> flips;
+        void* table = nullptr;
+    };
+    std::vector<PTab> ptabs;
 };
 
 static void drop_group_graphs(gkm_group* g) {
@@ -522,6 +529,9 @@ static void drop_group_graphs(gkm_group* g) {
         if (c.graph) (void)hipGraphDestroy(c.graph);
     }
     g->caps.clear();
+    for (auto& t : g->ptabs)
+        if (t.table) (void)hipFree(t.table);
+    g->ptabs.clear();
 }
 
 static void free_group(gkm_group* g) {
@@ -644,6 +654,94 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
             HIP_TRY(hipGraphLaunch(cap->exec, g->stream));
             for (Ctx* h : g->hs) gkm::swap_state(h);
         }
+    }
+    // the handles' later work follows the group's; ghosts filled for reads
+    HIP_TRY(hipEventRecord(g->done, g->stream));
+    for (Ctx* h : g->hs) {
+        HIP_TRY(hipStreamWaitEvent(h->stream, g->done, 0));
+        gkm::ensure_ghosts(h);
+        if (nsteps > 0) {
+            int rc = queue_flag_copy(h);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return GKM_OK;
+}
+
+// One cooperative launch of the persistent kernel (k_group_persist) advances
+// every box of every handle nsteps steps.  The handles must share the
+// precision, reconstruction and energy mode, run the WENO-BGK scheme and keep
+// the periodic fill; the per-stage path (gkm_group_step) has no such limits.
+int gkm_group_step_persistent(gkm_group_t* g, int nsteps) {
+    if (!g) return fail(GKM_E_ARG, "null group");
+    if (nsteps < 0) return fail(GKM_E_ARG, "nsteps must be >= 0");
+    const int n = (int)g->hs.size();
+    if (n > gkm::kPersistMaxClasses)
+        return fail(GKM_E_ARG, "the persistent step takes at most %d handles (got %d)", gkm::kPersistMaxClasses, n);
+    const gkm_params& p0 = g->hs[0]->par;
+    for (int i = 0; i < n; i++) {
+        const gkm_params& p = g->hs[i]->par;
+        if (p.flux_scheme != GKM_SCHEME_WENO_BGK || g->hs[i]->halo_mode != GKM_HALO_GATHER)
+            return fail(GKM_E_ARG, "handle %d: the persistent step runs the WENO-BGK scheme with the periodic fill "
+                                   "(GKM_HALO_GATHER); use gkm_group_step", i);
+        if (p.precision != p0.precision || p.recon != p0.recon || p.energy_recon != p0.energy_recon)
+            return fail(GKM_E_ARG, "handle %d: the persistent step needs one precision, reconstruction and energy "
+                                   "mode across the group; use gkm_group_step", i);
+    }
+    // launch shapes changed since the tables were made (gkm_set_tuning)
+    std::vector<unsigned> gens(n);
+    for (int i = 0; i < n; i++) gens[i] = g->hs[i]->tune_gen;
+    if (gens != g->gens) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        drop_group_graphs(g);
+        g->gens = gens;
+    }
+    for (int i = 0; i < n; i++) {
+        Ctx* h = g->hs[i];
+        int rc = poll_physical(h);
+        if (rc) return rc;
+        std::vector<double> dt(h->nbox);
+        for (int b = 0; b < h->nbox; b++) {
+            dt[b] = h->h_pars[b].dt;
+            if (!(dt[b] > 0))
+                return fail(GKM_E_ARG, "handle %d box %d has no dt: call gkm_set_dt_batch or gkm_step_batch first", i,
+                            b);
+        }
+        if (!h->pars_valid) {
+            rc = upload_pars(h, dt.data());
+            if (rc) return rc;
+        }
+        gkm::prepare_step(h);
+        HIP_TRY(hipEventRecord(g->pre[i], h->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, g->pre[i], 0));
+    }
+    if (nsteps > 0) {
+        std::vector<int> flips(n);
+        for (int i = 0; i < n; i++) flips[i] = g->hs[i]->flip;
+        gkm_group::PTab* tab = nullptr;
+        for (auto& t : g->ptabs)
+            if (t.flips == flips) tab = &t;
+        if (!tab) {
+            const size_t one = gkm::persist_class_bytes(p0.precision);
+            std::vector<unsigned char> host(one * (size_t)n);
+            for (int i = 0; i < n; i++) gkm::persist_fill_class(g->hs[i], host.data() + one * (size_t)i);
+            gkm_group::PTab t;
+            t.flips = flips;
+            HIP_TRY(hipMalloc(&t.table, host.size()));
+            const hipError_t e = hipMemcpy(t.table, host.data(), host.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void)hipFree(t.table);
+                return fail(GKM_E_HIP, "class table copy failed: %s", hipGetErrorString(e));
+            }
+            g->ptabs.push_back(t);
+            tab = &g->ptabs.back();
+        }
+        int grid = 0;
+        const hipError_t e = gkm::launch_persist(p0, tab->table, n, nsteps, g->stream, &grid);
+        if (e != hipSuccess) return fail(GKM_E_HIP, "persistent launch failed: %s", hipGetErrorString(e));
+        for (Ctx* h : g->hs)
+            for (int s = 0; s < nsteps; s++) gkm::swap_state(h);
     }
     // the handles' later work follows the group's; ghosts filled for reads
     HIP_TRY(hipEventRecord(g->done, g->stream));

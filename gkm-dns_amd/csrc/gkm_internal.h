@@ -114,6 +114,14 @@ void ensure_ghosts(Ctx* h);             // ghosts valid for a read of the state
 // buffer and swaps the current-state pointers (swap_state)
 void launch_step(Ctx* h, hipEvent_t* ev);  // ev: kEvPerStep events or nullptr
 void swap_state(Ctx* h);
+// persistent group step: the size of one class entry, an entry made from a
+// handle's current buffers, and the cooperative launch over ncls entries
+// (device table) for nsteps steps; grid_out receives the block count
+constexpr int kPersistMaxClasses = 16;
+size_t persist_class_bytes(int precision);
+void persist_fill_class(Ctx* h, void* dst);
+hipError_t launch_persist(const gkm_params& par, const void* table, int ncls, int nsteps, hipStream_t s,
+                          int* grid_out);
 void launch_split(Ctx* h, int box, int q, const double* dsrc);
 void launch_merge(Ctx* h, int box, int q, double* ddst);
 void launch_flux_points(hipStream_t s, int prec, int n, const double* rec, double dx, double gam, double invPr, int K,

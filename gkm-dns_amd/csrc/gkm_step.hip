@@ -2,6 +2,7 @@
 // launchers: periodic ghost fill fused with the per-cell primitives, WENO
 // reconstruction per sweep, BGK flux per sweep, conservative update, state
 // split/merge for the mixed form.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -188,13 +189,15 @@ __device__ __forceinline__ void recon_one_carry(const Real (&v)[6][6], Real eps,
 
 // Unit-stride sweep (Z): one interface per thread; neighbouring threads share
 // their six-cell stencils through the cache.
+// (the block functions below take the block's place -- bx along the work, box
+// -- as arguments, so that the persistent group kernel can run them too)
 template <class Real, class St, int RECON, int ENERGY>
-__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st, const Real* __restrict__ Tw,
-                                               Planes10<Real> wi, const StepPar* __restrict__ pars) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void recon_z_block(const Geo& g, const Box3& b, unsigned sn, const St& st,
+                                              const Real* __restrict__ Tw, const Planes10<Real>& wi,
+                                              const StepPar& par, unsigned bx, unsigned box) {
+    const unsigned t = bx * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const StepPar& par = pars[blockIdx.y];
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
+    const unsigned c = box * g.ncell + box_index(g, b, t);
     Real v[6][6];
     constexpr int nlin = ENERGY == 1 ? 5 : 4;
 #pragma unroll
@@ -213,6 +216,12 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st
     }
 }
 
+template <class Real, class St, int RECON, int ENERGY>
+__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st, const Real* __restrict__ Tw,
+                                               Planes10<Real> wi, const StepPar* __restrict__ pars) {
+    recon_z_block<Real, St, RECON, ENERGY>(g, b, sn, st, Tw, wi, pars[blockIdx.y], blockIdx.x, blockIdx.y);
+}
+
 // Strided sweeps (X, Y): a thread walks CHUNK consecutive interfaces along the
 // sweep normal with a six-cell register window, so each cell is loaded once
 // per chunk instead of six times, and carries the smoothness weights of each
@@ -220,19 +229,19 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st
 // along z (unit stride) for coalescing.  Box axes: n0 = chunks, n1 = the other
 // tangential axis, n2 = z.
 template <class Real, class St, int RECON, int ENERGY, int CHUNK>
-__global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, unsigned sa, int n_normal, St st,
-                                                   const Real* __restrict__ Tw, Planes10<Real> wi,
-                                                   const StepPar* __restrict__ pars) {
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void recon_win_block(const Geo& g, const Box3& b, unsigned sn, unsigned sa, int n_normal,
+                                                const St& st, const Real* __restrict__ Tw,
+                                                const Planes10<Real>& wi, const StepPar& par, unsigned bx,
+                                                unsigned box) {
+    const unsigned t = bx * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
-    const StepPar& par = pars[blockIdx.y];
     const unsigned r = fdiv(t, b.f2);
     const unsigned k = t - r * (unsigned)b.n2;
     const unsigned chunk = fdiv(r, b.f1);
     const unsigned a = r - chunk * (unsigned)b.n1;
     const int first = (int)chunk * CHUNK;                 // normal index offset from 2
     const int count = min(CHUNK, n_normal - first);
-    unsigned c = blockIdx.y * g.ncell + (unsigned)(first + 2) * sn + (a + 2) * sa + (k + 2);
+    unsigned c = box * g.ncell + (unsigned)(first + 2) * sn + (a + 2) * sa + (k + 2);
     constexpr bool tmode = ENERGY == 0;
     constexpr int nlin = tmode ? 4 : 5;
     Real v[6][6];
@@ -282,6 +291,14 @@ __global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, u
         }
         c += sn;
     }
+}
+
+template <class Real, class St, int RECON, int ENERGY, int CHUNK>
+__global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, unsigned sa, int n_normal, St st,
+                                                   const Real* __restrict__ Tw, Planes10<Real> wi,
+                                                   const StepPar* __restrict__ pars) {
+    recon_win_block<Real, St, RECON, ENERGY, CHUNK>(g, b, sn, sa, n_normal, st, Tw, wi, pars[blockIdx.y], blockIdx.x,
+                                                    blockIdx.y);
 }
 
 // slot -> field map of the rotated flux frame (update.cpp:340-352, 429-441)
@@ -342,19 +359,25 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
 }
 
 template <class Real, class St, int DIR>
+__device__ __forceinline__ void flux_block(const Geo& g, const Box3& b, unsigned sn, unsigned st1, unsigned st2,
+                                           const St& st, const Real* __restrict__ tau, const Planes10<Real>& wi,
+                                           const Planes5<Real>& F, const StepPar& par, unsigned bx, unsigned box) {
+    const unsigned t = bx * blockDim.x + threadIdx.x;
+    if (t >= b.count()) return;
+    const unsigned c = box * g.ncell + box_index(g, b, t);
+    Real Fs[5];
+    flux_at<Real, St, DIR>(c, sn, st1, st2, st, tau, wi, par, Fs);
+#pragma unroll
+    for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
+}
+
+template <class Real, class St, int DIR>
 __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
            Planes10<Real> wi, Planes5<Real> F, const StepPar* __restrict__ pars) {
     // plain block order: the XCD-contiguous numbering halves this kernel's
     // fetches but measured 4-5 % slower (the kernel is VALU-bound)
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count()) return;
-    const StepPar& par = pars[blockIdx.y];
-    const unsigned c = blockIdx.y * g.ncell + box_index(g, b, t);
-    Real Fs[5];
-    flux_at<Real, St, DIR>(c, sn, st1, st2, st, tau, wi, par, Fs);
-#pragma unroll
-    for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
+    flux_block<Real, St, DIR>(g, b, sn, st1, st2, st, tau, wi, F, pars[blockIdx.y], blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,16 +638,19 @@ __device__ __forceinline__ void zline_column(const ZLines& zl, unsigned l, unsig
     i = s * zl.fW.d + (r - j * fw.d);
 }
 
+// vb: the block's number along the box's wavefronts; lds: the block's dynamic
+// LDS (UpdSlots<St>::n values, then one image mask, per thread)
 template <class Real, class St, int SCHEME, bool GHOST>
-__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
-    k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
-                  Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars, Real* __restrict__ Tw_nx,
-                  Real* __restrict__ tau_nx, unsigned* __restrict__ bad) {
-    const unsigned gt = xcd_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flux_update_block(const Geo& g, const ZLines& zl, unsigned st1, unsigned st2,
+                                                  const St& st, const St& nx, const Real* __restrict__ tau,
+                                                  const Planes10<Real>& wi, const Planes15<Real>& F,
+                                                  const StepPar& par, Real* __restrict__ Tw_nx,
+                                                  Real* __restrict__ tau_nx, unsigned* __restrict__ bad,
+                                                  unsigned vb, unsigned box, unsigned char* upd_lds_raw) {
+    const unsigned gt = vb * blockDim.x + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned e = (gt >> 6) * 63u + lane;
     const bool valid = e < zl.count;
-    const StepPar& par = pars[blockIdx.y];
     unsigned c = 0, imask = 0;
     bool upd = false;  // lane 0's cell belongs to the previous wavefront; k = 2 is a ghost
     if (valid) {
@@ -633,7 +659,7 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
         upd = lane != 0 && kk != 0;
         unsigned j, i;
         zline_column(zl, line, j, i);
-        c = blockIdx.y * g.ncell + (j + 3) * g.s0 + (i + 3) * g.s1 + (kk + 2);
+        c = box * g.ncell + (j + 3) * g.s0 + (i + 3) * g.s1 + (kk + 2);
         if constexpr (GHOST) imask = image_mask(g, j + 3, i + 3, kk + 2);
     }
     // the flags as a wave-uniform mask (scalar registers), taken before the
@@ -641,7 +667,6 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     // across it
     const unsigned long long upd_mask = __ballot(upd);
     // UpdSlots<St>::n * blockDim.x values, then blockDim.x image masks
-    extern __shared__ unsigned char upd_lds_raw[];
     Real* upd_lds = reinterpret_cast<Real*>(upd_lds_raw);
     unsigned* img_lds = reinterpret_cast<unsigned*>(upd_lds + UpdSlots<St>::n * blockDim.x);
     // the image mask waits in LDS across the flux, like the update's operands
@@ -678,7 +703,139 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
         if constexpr (GHOST) ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
         ok = physical<St>(par, v);
     }
-    if (!ok) bad[blockIdx.y] = 1u;
+    if (!ok) bad[box] = 1u;
+}
+
+template <class Real, class St, int SCHEME, bool GHOST>
+__global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
+    k_flux_update(Geo g, ZLines zl, unsigned st1, unsigned st2, St st, St nx, const Real* __restrict__ tau,
+                  Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars, Real* __restrict__ Tw_nx,
+                  Real* __restrict__ tau_nx, unsigned* __restrict__ bad) {
+    extern __shared__ unsigned char upd_lds_raw[];
+    flux_update_block<Real, St, SCHEME, GHOST>(g, zl, st1, st2, st, nx, tau, wi, F, pars[blockIdx.y], Tw_nx, tau_nx,
+                                               bad, xcd_block(), blockIdx.y, upd_lds_raw);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent group step (gkm_group_step_persistent).  One cooperative launch
+// advances every box of every member handle (one size class each) nsteps
+// steps.  The grid holds as many blocks of 256 threads as fit on the chip at
+// once; every block runs the six phases of a step -- WENO X, flux X, WENO Y,
+// flux Y, WENO Z, fused Z flux + update + next ghosts -- over the phase's
+// blocks of every class and box in turn (the same block functions as the
+// per-stage kernels), with a grid barrier after each phase: a phase reads
+// what the one before wrote anywhere in the box (interface planes, X/Y
+// fluxes, the new state).  The state and tau buffers alternate each step as
+// the handles' do.  Every block reaches every barrier: the phase loops are
+// block-uniform and nothing returns early from the kernel.
+// ---------------------------------------------------------------------------
+template <class Real, class St> struct PClass {
+    Geo g;
+    int nbox, chunk;
+    unsigned nblk[6];                // blocks of 256 threads per box, per phase
+    Box3 rb[2];                      // WENO X, Y: chunks x other tangential x z
+    unsigned rsn[2], rsa[2];
+    int rnn[2];
+    Box3 zb;                         // WENO Z: the extended interface set
+    Box3 fb[2];                      // flux X, Y: the core interface set
+    unsigned fsn[2], fst1[2], fst2[2];
+    ZLines zl;                       // fused Z kernel
+    unsigned zst1, zst2;
+    St st[2];                        // state buffers; st[0] current at launch
+    Real* tau[2];
+    Real* Tw;
+    Planes10<Real> wi;
+    Planes15<Real> F;
+    const StepPar* pars;
+    unsigned* bad;
+};
+
+constexpr int kPersistThreads = 256;
+
+template <class Real, class St, int RECON, int ENERGY>
+__device__ __forceinline__ void persist_phase_block(const PClass<Real, St>& C, int ph, int par, unsigned bx,
+                                                    unsigned box, unsigned char* lds) {
+    const St& cur = C.st[par];
+    const St& nxt = C.st[par ^ 1];
+    const StepPar& sp = C.pars[box];
+    switch (ph) {
+    case 0:
+    case 2: {
+        const int d = ph >> 1;
+        if (C.chunk == 4)
+            recon_win_block<Real, St, RECON, ENERGY, 4>(C.g, C.rb[d], C.rsn[d], C.rsa[d], C.rnn[d], cur, C.Tw, C.wi,
+                                                        sp, bx, box);
+        else if (C.chunk == 16)
+            recon_win_block<Real, St, RECON, ENERGY, 16>(C.g, C.rb[d], C.rsn[d], C.rsa[d], C.rnn[d], cur, C.Tw, C.wi,
+                                                         sp, bx, box);
+        else
+            recon_win_block<Real, St, RECON, ENERGY, 8>(C.g, C.rb[d], C.rsn[d], C.rsa[d], C.rnn[d], cur, C.Tw, C.wi,
+                                                        sp, bx, box);
+        break;
+    }
+    case 1:
+    case 3: {
+        const int d = ph >> 1;
+        Planes5<Real> f;
+#pragma unroll
+        for (int q = 0; q < 5; q++) f.p[q] = C.F.p[5 * d + q];
+        if (d == 0)
+            flux_block<Real, St, 0>(C.g, C.fb[0], C.fsn[0], C.fst1[0], C.fst2[0], cur, C.tau[par], C.wi, f, sp, bx,
+                                    box);
+        else
+            flux_block<Real, St, 1>(C.g, C.fb[1], C.fsn[1], C.fst1[1], C.fst2[1], cur, C.tau[par], C.wi, f, sp, bx,
+                                    box);
+        break;
+    }
+    case 4:
+        recon_z_block<Real, St, RECON, ENERGY>(C.g, C.zb, 1u, cur, C.Tw, C.wi, sp, bx, box);
+        break;
+    default:
+        flux_update_block<Real, St, GKM_SCHEME_WENO_BGK, true>(C.g, C.zl, C.zst1, C.zst2, cur, nxt, C.tau[par], C.wi,
+                                                               C.F, sp, C.Tw, C.tau[par ^ 1], C.bad, bx, box, lds);
+        break;
+    }
+}
+
+template <class Real, class St, int RECON, int ENERGY>
+__global__ void __launch_bounds__(kPersistThreads, sizeof(Real) == 8 ? 2 : 1)
+    k_group_persist(const PClass<Real, St>* __restrict__ cls, int ncls, int nsteps) {
+    extern __shared__ unsigned char persist_lds[];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const unsigned G = gridDim.x, p = blockIdx.x;
+    for (int step = 0; step < nsteps; step++) {
+        const int par = step & 1;
+        for (int ph = 0; ph < 6; ph++) {
+            unsigned V = 0;  // the phase's blocks over every class and box
+            for (int k = 0; k < ncls; k++) V += cls[k].nblk[ph] * (unsigned)cls[k].nbox;
+            // the fused Z phase takes consecutive blocks on one XCD (its lines'
+            // neighbours then share that XCD's L2, as xcd_block arranges for
+            // the per-stage kernel); the others take blocks round-robin
+            unsigned v0 = p, v1 = V, dv = G;
+            if (ph == 5) {
+                const unsigned x = p & 7u, q = V >> 3, rem = V & 7u;
+                v0 = x * q + min(x, rem) + (p >> 3);
+                v1 = x * q + min(x, rem) + q + (x < rem ? 1u : 0u);
+                dv = G >> 3;
+            }
+            for (unsigned v = v0; v < v1; v += dv) {
+                unsigned w = v;
+                int k = 0;
+                while (w >= cls[k].nblk[ph] * (unsigned)cls[k].nbox) {
+                    w -= cls[k].nblk[ph] * (unsigned)cls[k].nbox;
+                    k++;
+                }
+                const unsigned nb = cls[k].nblk[ph];
+                const unsigned box = w / nb, bx = w - box * nb;
+                persist_phase_block<Real, St, RECON, ENERGY>(cls[k], ph, par, bx, box, persist_lds);
+            }
+            // the grid barrier also makes the phase's stores visible to every
+            // thread of the grid (cooperative groups' sync); fences of our own
+            // on both sides of it measured 12 % slower (6.10 vs 5.43 ms per
+            // step on the 96-box ensemble) with the same bits
+            if (step + 1 < nsteps || ph < 5) grid.sync();
+        }
+    }
 }
 
 // ---- split / merge of the mixed state --------------------------------------
@@ -804,6 +961,21 @@ static inline int normal_axis(int dir) { return dir == 0 ? 1 : (dir == 1 ? 0 : 2
 
 static inline unsigned axis_stride(const Geo& g, int a) { return a == 0 ? g.s0 : (a == 1 ? g.s1 : 1u); }
 
+// z lines of the fused Z kernel, in strips of kZStrip columns of i
+static ZLines make_zlines(const Geo& g) {
+    ZLines zl;
+    const unsigned nI = (unsigned)g.nz + 1;
+    zl.count = (unsigned)g.ny * (unsigned)g.nx * nI;
+    zl.fnI = make_fastdiv(nI);
+    const unsigned W = std::min(kZStrip, (unsigned)g.nx);
+    zl.nfull = (unsigned)g.nx / W;
+    const unsigned wl = (unsigned)g.nx - zl.nfull * W;
+    zl.fstrip = make_fastdiv(W * (unsigned)g.ny);
+    zl.fW = make_fastdiv(W);
+    zl.fWl = make_fastdiv(wl ? wl : W);
+    return zl;
+}
+
 // tangential axes paired with the y / z slots (update.cpp:217-245, 291-318, 380-407)
 static inline void tangential_axes(int dir, int zbug, int& t1, int& t2) {
     if (dir == 0) { t1 = 0; t2 = 2; }
@@ -905,16 +1077,7 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     int t1, t2;
     tangential_axes(2, h->par.zbug_compat, t1, t2);
     const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
-    ZLines zl;
-    const unsigned nI = (unsigned)g.nz + 1;
-    zl.count = (unsigned)g.ny * (unsigned)g.nx * nI;
-    zl.fnI = make_fastdiv(nI);
-    const unsigned W = std::min(kZStrip, (unsigned)g.nx);
-    zl.nfull = (unsigned)g.nx / W;
-    const unsigned wl = (unsigned)g.nx - zl.nfull * W;
-    zl.fstrip = make_fastdiv(W * (unsigned)g.ny);
-    zl.fW = make_fastdiv(W);
-    zl.fWl = make_fastdiv(wl ? wl : W);
+    const ZLines zl = make_zlines(g);
     const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
     const int bs = h->zflux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
@@ -1103,6 +1266,116 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
     }
     if (ev) (void)hipEventRecord(ev[11], h->stream);
     swap_state(h);
+}
+
+// ---- persistent group step: class table and launch -------------------------
+template <class Real, class St>
+static void fill_pclass(Ctx* h, const St& cur, const St& nxt, Real* tau0, Real* tau1, Real* Tw, Real* const wi[10],
+                        Real* const F[15], PClass<Real, St>& C) {
+    std::memset(&C, 0, sizeof C);
+    const Geo& g = h->geo;
+    C.g = g;
+    C.nbox = h->nbox;
+    C.chunk = h->recon_chunk;
+    const int P[3] = {g.P0, g.P1, g.P2};
+    for (int d = 0; d < 2; d++) {  // WENO X (normal axis 1), Y (axis 0), as launch_recon forms them
+        const int an = normal_axis(d), oa = an == 1 ? 0 : 1;
+        const int n_normal = P[an] - 5;
+        const int chunks = (n_normal + C.chunk - 1) / C.chunk;
+        C.rb[d] = make_box(0, 0, 0, chunks, P[oa] - 4, P[2] - 4);
+        C.rsn[d] = axis_stride(g, an);
+        C.rsa[d] = axis_stride(g, oa);
+        C.rnn[d] = n_normal;
+        C.nblk[2 * d] = blocks_for(C.rb[d].count(), kPersistThreads);
+        int t1, t2;  // flux X, Y, as launch_sweep forms it
+        tangential_axes(d, h->par.zbug_compat, t1, t2);
+        C.fb[d] = sweep_box(g, an, false);
+        C.fsn[d] = axis_stride(g, an);
+        C.fst1[d] = axis_stride(g, t1);
+        C.fst2[d] = axis_stride(g, t2);
+        C.nblk[2 * d + 1] = blocks_for(C.fb[d].count(), kPersistThreads);
+    }
+    C.zb = sweep_box(g, 2, true);
+    C.nblk[4] = blocks_for(C.zb.count(), kPersistThreads);
+    C.zl = make_zlines(g);
+    int t1, t2;
+    tangential_axes(2, h->par.zbug_compat, t1, t2);
+    C.zst1 = axis_stride(g, t1);
+    C.zst2 = axis_stride(g, t2);
+    C.nblk[5] = blocks_for((C.zl.count - 1 + 62) / 63 * 64u, kPersistThreads);
+    C.st[0] = cur;
+    C.st[1] = nxt;
+    C.tau[0] = tau0;
+    C.tau[1] = tau1;
+    C.Tw = Tw;
+    for (int q = 0; q < 10; q++) C.wi.p[q] = wi[q];
+    for (int q = 0; q < 15; q++) C.F.p[q] = F[q];
+    C.pars = h->d_pars;
+    C.bad = h->d_bad;
+}
+
+size_t persist_class_bytes(int precision) {
+    return precision == GKM_PREC_FP64 ? sizeof(PClass<double, StateD>) : sizeof(PClass<float, StateM>);
+}
+
+void persist_fill_class(Ctx* h, void* dst) {
+    if (h->par.precision == GKM_PREC_FP64) {
+        StateD cur, nxt;
+        for (int q = 0; q < 5; q++) {
+            cur.w[q] = h->Wd[q];
+            nxt.w[q] = h->Wd_nx[q];
+        }
+        fill_pclass<double, StateD>(h, cur, nxt, h->taud, h->taud_nx, h->Td, h->WId, h->Fd,
+                                    *(PClass<double, StateD>*)dst);
+    } else {
+        StateM cur, nxt;
+        for (int q = 0; q < 5; q++) {
+            cur.hi[q] = h->Whi[q];
+            cur.lo[q] = h->Wlo[q];
+            nxt.hi[q] = h->Whi_nx[q];
+            nxt.lo[q] = h->Wlo_nx[q];
+        }
+        fill_pclass<float, StateM>(h, cur, nxt, h->tauf, h->tauf_nx, h->Tf, h->WIf, h->Ff,
+                                   *(PClass<float, StateM>*)dst);
+    }
+}
+
+template <class Real, class St, int RECON, int ENERGY>
+static hipError_t launch_persist_t(const void* table, int ncls, int nsteps, hipStream_t s, int* grid_out) {
+    const void* fn = reinterpret_cast<const void*>(&k_group_persist<Real, St, RECON, ENERGY>);
+    const size_t lds = (sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)) * kPersistThreads;
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_persist<Real, St, RECON, ENERGY>,
+                                                         kPersistThreads, lds);
+    if (e != hipSuccess) return e;
+    if (!coop) return hipErrorNotSupported;
+    // every block resident at once (the barriers wait for all of them); a
+    // multiple of 8, one share per XCD
+    const int G = per_cu * ncu / 8 * 8;
+    if (G < 8) return hipErrorInvalidConfiguration;
+    if (grid_out) *grid_out = G;
+    const PClass<Real, St>* cls = static_cast<const PClass<Real, St>*>(table);
+    void* args[] = {(void*)&cls, (void*)&ncls, (void*)&nsteps};
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(kPersistThreads), args, (unsigned)lds, s);
+}
+
+hipError_t launch_persist(const gkm_params& par, const void* table, int ncls, int nsteps, hipStream_t s,
+                          int* grid_out) {
+    const bool js = par.recon == GKM_RECON_JS, en = par.energy_recon == GKM_ENERGY_E;
+    if (par.precision == GKM_PREC_FP64) {
+        if (js && en) return launch_persist_t<double, StateD, 1, 1>(table, ncls, nsteps, s, grid_out);
+        if (js) return launch_persist_t<double, StateD, 1, 0>(table, ncls, nsteps, s, grid_out);
+        if (en) return launch_persist_t<double, StateD, 0, 1>(table, ncls, nsteps, s, grid_out);
+        return launch_persist_t<double, StateD, 0, 0>(table, ncls, nsteps, s, grid_out);
+    }
+    if (js && en) return launch_persist_t<float, StateM, 1, 1>(table, ncls, nsteps, s, grid_out);
+    if (js) return launch_persist_t<float, StateM, 1, 0>(table, ncls, nsteps, s, grid_out);
+    if (en) return launch_persist_t<float, StateM, 0, 1>(table, ncls, nsteps, s, grid_out);
+    return launch_persist_t<float, StateM, 0, 0>(table, ncls, nsteps, s, grid_out);
 }
 
 void launch_split(Ctx* h, int box, int q, const double* dsrc) {

@@ -36,7 +36,8 @@ EXPORTS = (
     "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
     "gkm_print_stats", "gkm_print_stats_box", "gkm_step_print_stats", "gkm_format_print_stats", "gkm_gks_points",
     "gkm_last_sweep_timing", "gkm_get_grid", "gkm_get_params_box", "gkm_set_params_box",
-    "gkm_set_dt_batch", "gkm_group_create", "gkm_group_step", "gkm_group_sync", "gkm_group_destroy",
+    "gkm_set_dt_batch", "gkm_group_create", "gkm_group_step", "gkm_group_step_persistent", "gkm_group_sync",
+    "gkm_group_destroy",
     "gkm_flux_points_noalloc",
     # include/gkm_ic.h
     "gkm_dhit_velocity", "gkm_init_dhit_box", "gkm_init_dhit",

@@ -9,8 +9,10 @@ captured hipGraph on its own stream, so the size classes run concurrently.
 With group=True one captured graph holds a step of every size class instead
 (gkm_group_step), so one graph launch advances the whole ensemble; on MI355X
 that measured ~3 % slower than the per-class streams at the 96-box baseline
-ensemble, so it is not the default.  Launch shapes per size come from the
-autotuner (tune.py).
+ensemble, so it is not the default.  use_group(persistent=True) advances the
+whole ensemble by one persistent cooperative launch per call to step(), all
+steps included (gkm_group_step_persistent; same results bit for bit).  Launch
+shapes per size come from the autotuner (tune.py).
 """
 from __future__ import annotations
 
@@ -64,14 +66,17 @@ class Ensemble:
         # group: one captured graph holds a step of every size class, so one
         # graph launch advances the whole ensemble (gkm_group_step)
         self.group = Group(list(self.batches.values())) if group else None
+        self.persistent = False
 
-    def use_group(self, on: bool = True) -> None:
-        """Step through one group graph (True) or each batch on its own stream."""
+    def use_group(self, on: bool = True, persistent: bool = False) -> None:
+        """Step through one group graph per step (on), one persistent launch
+        per step() call (on, persistent), or each batch on its own stream."""
         if on and self.group is None:
             self.group = Group(list(self.batches.values()))
         elif not on and self.group is not None:
             self.group.close()
             self.group = None
+        self.persistent = bool(on and persistent)
 
     def timestep(self, cfl: float) -> dict[int, np.ndarray]:
         return {n: b.timestep(cfl) for n, b in self.batches.items()}
@@ -82,7 +87,7 @@ class Ensemble:
 
     def step(self, nsteps: int = 1) -> None:
         if self.group is not None:
-            self.group.step(nsteps)
+            self.group.step(nsteps, persistent=self.persistent)
             return
         for b in self.batches.values():
             b.step(nsteps)

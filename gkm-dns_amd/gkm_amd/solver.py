@@ -447,11 +447,17 @@ class Group:
         arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
         _abi.check(self._L.gkm_group_create(arr, len(self.members), C.byref(self._g)))
 
-    def step(self, nsteps: int = 1) -> None:
+    def step(self, nsteps: int = 1, persistent: bool = False) -> None:
+        """nsteps steps of every member: one captured graph launch per step
+        (gkm_group_step), or with persistent=True one persistent cooperative
+        launch for all of them (gkm_group_step_persistent; same results)."""
         for m in self.members:  # each member's current dt (uploaded only when it changed)
             d = np.ascontiguousarray(m.dt if isinstance(m, Batch) else [m.info.dt], dtype=np.float64)
             _abi.check(self._L.gkm_set_dt_batch(m._h, d.ctypes.data_as(_abi.DP)))
-        _abi.check(self._L.gkm_group_step(self._g, int(nsteps)))
+        if persistent:
+            _abi.check(self._L.gkm_group_step_persistent(self._g, int(nsteps)))
+        else:
+            _abi.check(self._L.gkm_group_step(self._g, int(nsteps)))
         for m in self.members:
             if isinstance(m, Batch):
                 m.t += nsteps * m.dt

@@ -188,6 +188,17 @@ int gkm_set_graph(gkm_t* h, int enabled);
 typedef struct gkm_group gkm_group_t;
 int gkm_group_create(gkm_t* const* handles, int n, gkm_group_t** out);
 int gkm_group_step(gkm_group_t* g, int nsteps);
+/* The same steps as gkm_group_step, bit for bit, from one persistent launch:
+ * a cooperative kernel whose grid fits on the device at once runs the six
+ * phases of every step (WENO and flux X, WENO and flux Y, WENO Z, fused Z flux
+ * + update) over every box of every handle, with a grid-wide barrier between
+ * phases, for all nsteps steps.  The handles must share precision,
+ * reconstruction and energy mode and use the WENO-BGK scheme with
+ * GKM_HALO_GATHER (GKM_E_ARG otherwise); at most 16 handles.  The kernel's
+ * blocks are 256 threads whatever gkm_set_tuning chose for the flux blocks
+ * (its WENO chunk is honoured).  No reference counterpart (the reference runs
+ * one realization per process, weno/src/Main.cpp:10-49). */
+int gkm_group_step_persistent(gkm_group_t* g, int nsteps);
 int gkm_group_sync(gkm_group_t* g);
 int gkm_group_destroy(gkm_group_t* g);
 
