@@ -486,6 +486,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_flux (BGK flux, X and Y sweeps)", "kernel_ms": flux_ms,
+                     "note": "the kernel is bound by VALU issue, not HBM (roofline_valu); bytes are "
+                             "the traffic-minimal algorithmic ones",
                      "alg_bytes_per_launch": alg_bytes_launch,
                      "step_frac": step_hbm / HBM_PEAK_GBS},
         "roofline_valu": valu,
