@@ -319,10 +319,31 @@ template <int DIR> __device__ __forceinline__ int slot_field(int s) {
 // a bound; forcing 4 spills.
 // ---------------------------------------------------------------------------
 
+// The transverse slope differences of interface c into per-thread LDS slots,
+// slot k of thread t at lds[k * blockDim.x + t]: Dy left (k = s), Dy right
+// (5 + s), Dz left (10 + s), Dz right (15 + s), in the rotated slot order.  The
+// FP64 flux stages them ahead of its arithmetic, so their forty loads are all
+// in flight at once and overlap the other wave's work instead of stalling
+// this one in the middle of the flux (the values are the same differences).
+template <class Real, int DIR>
+__device__ __forceinline__ void flux_stage_slopes(unsigned c, unsigned st1, unsigned st2, const Planes10<Real>& wi,
+                                                  Real* lds) {
+    const unsigned t = threadIdx.x, b = blockDim.x;
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int q = slot_field<DIR>(s);
+        lds[s * b + t] = ld(wi.p[q], c + st1) - ld(wi.p[q], c - st1);
+        lds[(5 + s) * b + t] = ld(wi.p[5 + q], c + st1) - ld(wi.p[5 + q], c - st1);
+        lds[(10 + s) * b + t] = ld(wi.p[q], c + st2) - ld(wi.p[q], c - st2);
+        lds[(15 + s) * b + t] = ld(wi.p[5 + q], c + st2) - ld(wi.p[5 + q], c - st2);
+    }
+}
+
+// staged: the transverse differences come from flux_stage_slopes' slots
 template <class Real, class St, int DIR>
 __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, unsigned st2, const St& st,
                                         const Real* __restrict__ tau, const Planes10<Real>& wi,
-                                        const StepPar& par, Real Fs[5]) {
+                                        const StepPar& par, Real Fs[5], const Real* staged = nullptr) {
     Gas<Real> gas;
     gas.gam = upar<Real>(par, kUGam);
     gas.invPr = upar<Real>(par, kUInvPr);
@@ -351,8 +372,14 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
             // differences; the flux scales them by 2/dx (normal: half-cell
             // distance) and 1/(2 dx) (tangential: central difference)
             Dx[s] = side == 0 ? Wl[s] - WL[s] : WR[s] - Wr[s];
-            Dy[s] = ld(wi.p[o], c + st1) - ld(wi.p[o], c - st1);
-            Dz[s] = ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2);
+            if (staged) {
+                const unsigned t = threadIdx.x, b = blockDim.x;
+                Dy[s] = staged[(5 * side + s) * b + t];
+                Dz[s] = staged[(10 + 5 * side + s) * b + t];
+            } else {
+                Dy[s] = ld(wi.p[o], c + st1) - ld(wi.p[o], c - st1);
+                Dz[s] = ld(wi.p[o], c + st2) - ld(wi.p[o], c - st2);
+            }
         }
     };
     bgk_flux(gas, WL, WR, Wl, Wr, slopes, idx2, hidx, tau0, upar<Real>(par, kUDt), upar<Real>(par, kUDx), Fs);
@@ -361,12 +388,14 @@ __device__ __forceinline__ void flux_at(unsigned c, unsigned sn, unsigned st1, u
 template <class Real, class St, int DIR>
 __device__ __forceinline__ void flux_block(const Geo& g, const Box3& b, unsigned sn, unsigned st1, unsigned st2,
                                            const St& st, const Real* __restrict__ tau, const Planes10<Real>& wi,
-                                           const Planes5<Real>& F, const StepPar& par, unsigned bx, unsigned box) {
+                                           const Planes5<Real>& F, const StepPar& par, unsigned bx, unsigned box,
+                                           Real* stage_lds = nullptr) {
     const unsigned t = bx * blockDim.x + threadIdx.x;
     if (t >= b.count()) return;
     const unsigned c = box * g.ncell + box_index(g, b, t);
+    if (stage_lds) flux_stage_slopes<Real, DIR>(c, st1, st2, wi, stage_lds);
     Real Fs[5];
-    flux_at<Real, St, DIR>(c, sn, st1, st2, st, tau, wi, par, Fs);
+    flux_at<Real, St, DIR>(c, sn, st1, st2, st, tau, wi, par, Fs, stage_lds);
 #pragma unroll
     for (int s = 0; s < 5; s++) put(F.p[slot_field<DIR>(s)], c, Fs[s]);
 }
@@ -376,8 +405,11 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
     k_flux(Geo g, Box3 b, unsigned sn, unsigned st1, unsigned st2, St st, const Real* __restrict__ tau,
            Planes10<Real> wi, Planes5<Real> F, const StepPar* __restrict__ pars) {
     // plain block order: the XCD-contiguous numbering halves this kernel's
-    // fetches but measured 4-5 % slower (the kernel is VALU-bound)
-    flux_block<Real, St, DIR>(g, b, sn, st1, st2, st, tau, wi, F, pars[blockIdx.y], blockIdx.x, blockIdx.y);
+    // fetches but measured 4-5 % slower (the kernel is VALU-bound).  FP64
+    // stages the transverse slope differences in LDS (flux_stage_slopes).
+    extern __shared__ unsigned char flux_lds_raw[];
+    Real* stage = sizeof(Real) == 8 ? reinterpret_cast<Real*>(flux_lds_raw) : nullptr;
+    flux_block<Real, St, DIR>(g, b, sn, st1, st2, st, tau, wi, F, pars[blockIdx.y], blockIdx.x, blockIdx.y, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -712,6 +744,8 @@ __global__ void __launch_bounds__(256, sizeof(Real) == 8 ? 2 : 1)
                   Planes10<Real> wi, Planes15<Real> F, const StepPar* __restrict__ pars, Real* __restrict__ Tw_nx,
                   Real* __restrict__ tau_nx, unsigned* __restrict__ bad) {
     extern __shared__ unsigned char upd_lds_raw[];
+    // (staging the transverse slope differences here too, as k_flux does in
+    // FP64, measured 1.56 -> 1.74 ms: it brings back a 36-byte spill)
     flux_update_block<Real, St, SCHEME, GHOST>(g, zl, st1, st2, st, nx, tau, wi, F, pars[blockIdx.y], Tw_nx, tau_nx,
                                                bad, xcd_block(), blockIdx.y, upd_lds_raw);
 }
@@ -779,12 +813,14 @@ __device__ __forceinline__ void persist_phase_block(const PClass<Real, St>& C, i
         Planes5<Real> f;
 #pragma unroll
         for (int q = 0; q < 5; q++) f.p[q] = C.F.p[5 * d + q];
+        // as k_flux: FP64 stages the transverse slope differences (the same code, so the same bits)
+        Real* stage = sizeof(Real) == 8 ? reinterpret_cast<Real*>(lds) : nullptr;
         if (d == 0)
             flux_block<Real, St, 0>(C.g, C.fb[0], C.fsn[0], C.fst1[0], C.fst2[0], cur, C.tau[par], C.wi, f, sp, bx,
-                                    box);
+                                    box, stage);
         else
             flux_block<Real, St, 1>(C.g, C.fb[1], C.fsn[1], C.fst1[1], C.fst2[1], cur, C.tau[par], C.wi, f, sp, bx,
-                                    box);
+                                    box, stage);
         break;
     }
     case 4:
@@ -1057,11 +1093,12 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     // WENO-BGK: the X and Y sweeps (the Z sweep's flux is fused with the update)
     launch_recon_any<Real, St>(h, dir, st, Tw, wi);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
+    const size_t lds = sizeof(Real) == 8 ? 20 * sizeof(Real) * (size_t)bs : 0;  // flux_stage_slopes
     if (dir == 0)
-        hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+        hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), lds, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
     else
-        hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
+        hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), lds, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
 }
@@ -1343,7 +1380,11 @@ void persist_fill_class(Ctx* h, void* dst) {
 template <class Real, class St, int RECON, int ENERGY>
 static hipError_t launch_persist_t(const void* table, int ncls, int nsteps, hipStream_t s, int* grid_out) {
     const void* fn = reinterpret_cast<const void*>(&k_group_persist<Real, St, RECON, ENERGY>);
-    const size_t lds = (sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)) * kPersistThreads;
+    // the fused Z phase's update slots and image masks, or the FP64 flux
+    // phases' 20 staged slope differences per thread, whichever is larger
+    const size_t lds = std::max((sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)),
+                                sizeof(Real) == 8 ? 20 * sizeof(Real) : (size_t)0) *
+                       kPersistThreads;
     int dev = 0, coop = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
