@@ -222,6 +222,62 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st
     recon_z_block<Real, St, RECON, ENERGY>(g, b, sn, st, Tw, wi, pars[blockIdx.y], blockIdx.x, blockIdx.y);
 }
 
+// The Z sweep with an LDS row tile.  A block's consecutive interfaces lie on
+// consecutive z lines of one j row, which are contiguous in memory, so the
+// cells its stencils read form one range [c_first - 2, c_last + 3]: the block
+// loads that range once (coalesced, each cell once) and reads the six-cell
+// stencils from LDS.  Inside a row, 256 consecutive interfaces cross at most
+// ceil(255 / nI) line ends and each crossing skips 5 cells, so the range is at
+// most span_max = 261 + 5 ceil(255 / nI) cells; a block whose range is longer
+// (it spans two j rows) loads from memory like k_recon.
+template <class Real, class St, int RECON, int ENERGY>
+__global__ void __launch_bounds__(256) k_recon_zt(Geo g, Box3 b, St st, const Real* __restrict__ Tw,
+                                                  Planes10<Real> wi, const StepPar* __restrict__ pars,
+                                                  unsigned span_max) {
+    extern __shared__ unsigned char zt_raw[];
+    Real* tile = reinterpret_cast<Real*>(zt_raw);  // [6][span_max]: fields 0..4 (or 0..3 and T in slot 5)
+    constexpr int nlin = ENERGY == 1 ? 5 : 4;
+    const unsigned n = b.count();
+    const unsigned t0 = blockIdx.x * blockDim.x, t = t0 + threadIdx.x;
+    const unsigned box0 = blockIdx.y * g.ncell;
+    const unsigned lo = box_index(g, b, t0) - 2u;
+    const unsigned span = box_index(g, b, min(t0 + blockDim.x, n) - 1u) + 3u - lo + 1u;
+    const bool tiled = span <= span_max;  // block-uniform
+    if (tiled) {
+        for (unsigned z = threadIdx.x; z < span; z += blockDim.x) {
+            const unsigned cc = box0 + lo + z;
+#pragma unroll
+            for (int q = 0; q < nlin; q++) tile[q * span_max + z] = (Real)st.cell(q, cc);
+            if (ENERGY == 0) tile[5 * span_max + z] = ld(Tw, cc);
+        }
+        __syncthreads();
+    }
+    if (t >= n) return;
+    if (!tiled) {
+        recon_z_block<Real, St, RECON, ENERGY>(g, b, 1u, st, Tw, wi, pars[blockIdx.y], blockIdx.x, blockIdx.y);
+        return;
+    }
+    const StepPar& par = pars[blockIdx.y];
+    const unsigned cl = box_index(g, b, t);
+    const unsigned o = cl - lo - 2u;  // the stencil's first cell in the tile
+    Real v[6][6];
+#pragma unroll
+    for (int q = 0; q < nlin; q++)
+#pragma unroll
+        for (int s = 0; s < 6; s++) v[q][s] = tile[q * span_max + o + s];
+    if (ENERGY == 0)
+#pragma unroll
+        for (int s = 0; s < 6; s++) v[5][s] = tile[5 * span_max + o + s];
+    Real wl[5], wr[5];
+    recon_one<RECON, ENERGY>(v, Real(1e-10), par, wl, wr);
+    const unsigned c = box0 + cl;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        put(wi.p[q], c, wl[q]);
+        put(wi.p[5 + q], c, wr[q]);
+    }
+}
+
 // Strided sweeps (X, Y): a thread walks CHUNK consecutive interfaces along the
 // sweep normal with a six-cell register window, so each cell is loaded once
 // per chunk instead of six times, and carries the smoothness weights of each
@@ -1034,8 +1090,11 @@ static void launch_recon(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> 
     const unsigned sn = axis_stride(g, an);
     if (dir == 2) {
         Box3 be = sweep_box(g, an, true);
-        hipLaunchKernelGGL((k_recon<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256), h->nbox), dim3(256),
-                           0, h->stream, g, be, sn, st, Tw, wi, h->d_pars);
+        const unsigned nI = (unsigned)be.n2;
+        const unsigned span_max = 261u + 5u * ((255u + nI - 1u) / nI);
+        const size_t lds = 6 * sizeof(Real) * (size_t)span_max;
+        hipLaunchKernelGGL((k_recon_zt<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256), h->nbox),
+                           dim3(256), lds, h->stream, g, be, st, Tw, wi, h->d_pars, span_max);
         return;
     }
     // X (normal = axis 1) walks i with threads over (j, k); Y (normal = axis 0) walks j over (i, k)
