@@ -395,6 +395,84 @@ __device__ __forceinline__ void acc_w(T acc[5], T s, const NTab<T>& t, T h, cons
     acc[4] += sh * a[3] * t.Qp[k];
 }
 
+// The flux's time-slope, time-source and flux moments are all of the form
+//   G_k = <u^k psi' (u ax + v' ay + w' az) . psi'> + <u^k psi' (sv . psi')>
+//       = u_{k+1}(ax) + v_k(ay) + w_k(az) + u_k(sv)         (acc_u/v/w above)
+// with sv = V ay + W az.  Summed component by component, the four terms share
+// their table entries, so G_k is five short dot products of the table with
+// coefficients formed once per (ax, ay, az, sv):
+//   G_k[0] = a0 N_k + a1 N_k+1 + a2 N_k+2 + a3 Q_k + a4 Q_k+1
+//   G_k[1] = the same one entry up (= G_k+1[0])
+//   G_k[2] = h (g2 N_k + d2 N_k+1 + ay4 Qp_k)
+//   G_k[3] = h (g3 N_k + d3 N_k+1 + az4 Qp_k)
+//   G_k[4] = sv0 Q_k + e Qp_k + a1 Q_k+1 + a2 Q_k+2 + a3 R_k + a4 R_k+1
+// where e = h (ay2 + az3), a0 = e + sv0, a1 = ax0 + sv1, a2 = ax1, a3 = sv4,
+// a4 = ax4, g2 = ay0 + sv2, d2 = ax2 + ay1, g3 = az0 + sv3, d3 = ax3 + az1.
+template <class T> struct GCoef {
+    T a0, a1, a2, a3, a4, s0, e, g2, d2, y4, g3, d3, z4, h;
+};
+
+template <class T>
+__device__ __forceinline__ void g_coef(const T ax[5], const T ay[5], const T az[5], const T sv[5], T h,
+                                       GCoef<T>& c) {
+    c.h = h;
+    c.e = h * (ay[2] + az[3]);
+    c.s0 = sv[0];
+    c.a0 = c.e + sv[0];
+    c.a1 = ax[0] + sv[1];
+    c.a2 = ax[1];
+    c.a3 = sv[4];
+    c.a4 = ax[4];
+    c.g2 = ay[0] + sv[2];
+    c.d2 = ax[2] + ay[1];
+    c.y4 = ay[4];
+    c.g3 = az[0] + sv[3];
+    c.d3 = ax[3] + az[1];
+    c.z4 = az[4];
+}
+
+// G_k alone
+template <int k, class T> __device__ __forceinline__ void g_mom(const NTab<T>& t, const GCoef<T>& c, T o[5]) {
+    o[0] = c.a0 * t.N[k] + c.a1 * t.N[k + 1] + c.a2 * t.N[k + 2] + c.a3 * t.Q[k] + c.a4 * t.Q[k + 1];
+    o[1] = c.a0 * t.N[k + 1] + c.a1 * t.N[k + 2] + c.a2 * t.N[k + 3] + c.a3 * t.Q[k + 1] + c.a4 * t.Q[k + 2];
+    o[2] = c.h * (c.g2 * t.N[k] + c.d2 * t.N[k + 1] + c.y4 * t.Qp[k]);
+    o[3] = c.h * (c.g3 * t.N[k] + c.d3 * t.N[k + 1] + c.z4 * t.Qp[k]);
+    o[4] = c.s0 * t.Q[k] + c.e * t.Qp[k] + c.a1 * t.Q[k + 1] + c.a2 * t.Q[k + 2] + c.a3 * t.R[k] +
+           c.a4 * t.R[k + 1];
+}
+
+// G_k and G_k+1, sharing G_k[1] = G_k+1[0]
+template <int k, class T>
+__device__ __forceinline__ void g_pair(const NTab<T>& t, const GCoef<T>& c, T o0[5], T o1[5]) {
+    g_mom<k>(t, c, o0);
+    o1[0] = o0[1];
+    o1[1] = c.a0 * t.N[k + 2] + c.a1 * t.N[k + 3] + c.a2 * t.N[k + 4] + c.a3 * t.Q[k + 2] + c.a4 * t.Q[k + 3];
+    o1[2] = c.h * (c.g2 * t.N[k + 1] + c.d2 * t.N[k + 2] + c.y4 * t.Qp[k + 1]);
+    o1[3] = c.h * (c.g3 * t.N[k + 1] + c.d3 * t.N[k + 2] + c.z4 * t.Qp[k + 1]);
+    o1[4] = c.s0 * t.Q[k + 1] + c.e * t.Qp[k + 1] + c.a1 * t.Q[k + 2] + c.a2 * t.Q[k + 3] + c.a3 * t.R[k + 1] +
+            c.a4 * t.R[k + 2];
+}
+
+// u_k(a) = <u^k psi' (a . psi')> alone and with u_k+1(a), sharing u_k[1] = u_k+1[0]
+template <int k, class T> __device__ __forceinline__ void u_mom(const NTab<T>& t, T h, const T a[5], T o[5]) {
+    o[0] = a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Q[k];
+    o[1] = a[0] * t.N[k + 1] + a[1] * t.N[k + 2] + a[4] * t.Q[k + 1];
+    const T hn = h * t.N[k];
+    o[2] = hn * a[2];
+    o[3] = hn * a[3];
+    o[4] = a[0] * t.Q[k] + a[1] * t.Q[k + 1] + a[4] * t.R[k];
+}
+template <int k, class T>
+__device__ __forceinline__ void u_pair(const NTab<T>& t, T h, const T a[5], T o0[5], T o1[5]) {
+    u_mom<k>(t, h, a, o0);
+    o1[0] = o0[1];
+    o1[1] = a[0] * t.N[k + 2] + a[1] * t.N[k + 3] + a[4] * t.Q[k + 2];
+    const T hn = h * t.N[k + 1];
+    o1[2] = hn * a[2];
+    o1[3] = hn * a[3];
+    o1[4] = a[0] * t.Q[k + 1] + a[1] * t.Q[k + 2] + a[4] * t.R[k + 1];
+}
+
 // moment / source vector: lab psi basis -> co-moving frame (Tinv)
 template <class T> __device__ __forceinline__ void to_frame(const T b[5], T V, T W, T o[5]) {
     o[0] = b[0];
@@ -404,100 +482,119 @@ template <class T> __device__ __forceinline__ void to_frame(const T b[5], T V, T
     o[4] = b[4] - V * b[2] - W * b[3] + T(0.5) * (V * V + W * W) * b[0];
 }
 
-// moment vector: co-moving frame -> lab psi basis (T), added to acc
+// moment vector: co-moving frame -> lab psi basis (T), added to acc (ADD) or
+// stored in it (the first contribution: IEEE arithmetic cannot drop a 0 + x)
+template <bool ADD, class T> __device__ __forceinline__ void to_lab(T acc[5], const T m[5], T V, T W) {
+    const T m2 = m[2] + V * m[0], m3 = m[3] + W * m[0];
+    const T m4 = m[4] + V * m[2] + W * m[3] + T(0.5) * (V * V + W * W) * m[0];
+    acc[0] = ADD ? acc[0] + m[0] : m[0];
+    acc[1] = ADD ? acc[1] + m[1] : m[1];
+    acc[2] = ADD ? acc[2] + m2 : m2;
+    acc[3] = ADD ? acc[3] + m3 : m3;
+    acc[4] = ADD ? acc[4] + m4 : m4;
+}
 template <class T> __device__ __forceinline__ void add_to_lab(T acc[5], const T m[5], T V, T W) {
-    acc[0] += m[0];
-    acc[1] += m[1];
-    acc[2] += m[2] + V * m[0];
-    acc[3] += m[3] + W * m[0];
-    acc[4] += m[4] + V * m[2] + W * m[3] + T(0.5) * (V * V + W * W) * m[0];
+    to_lab<true>(acc, m, V, W);
 }
 
-// slope solve in the co-moving frame (velocity (U, 0, 0)); b, a in psi'
+// Slope solve in the co-moving frame (velocity (U, 0, 0)); b, a in psi'
+// (GKMflux.cpp:280-296):
+//   R2 = b1 - U b0,  a4 = (1/Pr) (2 b4 - b0 q - 2 U R2) 4 lam^2 / (K+3),
+//   a3 = 2 lam b3,  a2 = 2 lam b2,  a1 = 2 lam R2 - U a4,  a0 = b0 - U a1 - a4 q / 2
+// with q = U^2 + (K+3)/(2 lam).  The Maxwellian's factors are formed once
+// (Slv); the solve is linear, so a scale s of the right-hand side is taken
+// into its two factors and b0 (solve_scaled) instead of into each b.
+template <class T> struct Slv {
+    T U, twoU, q, hq, k4, l2;  // U, 2U, q, q/2, 4 lam^2/(Pr (K+3)), 2 lam
+};
+
 template <class T>
-__device__ __forceinline__ void solve_frame(const Gas<T>& g, const T b[5], T U, T lam, T h, T a[5]) {
-    const T q = U * U + g.kp3 * h;  // U^2 + (K+3)/(2 lam)
-    const T R2 = b[1] - U * b[0];
-    a[4] = g.invPr * (2 * b[4] - b[0] * q - 2 * U * R2) * (4 * lam * lam) * g.ikp3;
-    a[3] = 2 * lam * b[3];
-    a[2] = 2 * lam * b[2];
-    a[1] = 2 * lam * R2 - U * a[4];
-    a[0] = b[0] - a[1] * U - T(0.5) * a[4] * q;
+__device__ __forceinline__ void slv_make(const Gas<T>& g, T U, T lam, T h, Slv<T>& S) {
+    S.U = U;
+    S.twoU = 2 * U;
+    S.q = U * U + g.kp3 * h;  // U^2 + (K+3)/(2 lam)
+    S.hq = T(0.5) * S.q;
+    S.l2 = 2 * lam;
+    S.k4 = (g.invPr * g.ikp3) * (S.l2 * S.l2);
+}
+
+// a = solve of s b, given k4s = s k4 and l2s = s l2
+template <class T>
+__device__ __forceinline__ void solve_scaled(const Slv<T>& S, T s, T k4s, T l2s, const T b[5], T a[5]) {
+    const T R2 = b[1] - S.U * b[0];
+    a[4] = k4s * (2 * b[4] - b[0] * S.q - S.twoU * R2);
+    a[3] = l2s * b[3];
+    a[2] = l2s * b[2];
+    a[1] = l2s * R2 - S.U * a[4];
+    a[0] = s * b[0] - S.U * a[1] - S.hq * a[4];
+}
+
+// a = solve of b
+template <class T> __device__ __forceinline__ void solve_frame(const Slv<T>& S, const T b[5], T a[5]) {
+    const T R2 = b[1] - S.U * b[0];
+    a[4] = S.k4 * (2 * b[4] - b[0] * S.q - S.twoU * R2);
+    a[3] = S.l2 * b[3];
+    a[2] = S.l2 * b[2];
+    a[1] = S.l2 * R2 - S.U * a[4];
+    a[0] = b[0] - S.U * a[1] - S.hq * a[4];
 }
 
 // Contribution of one side (left: normal table u > 0, right: u < 0).
 // Accumulates, in the lab frame: by, bz (sources of the equilibrium state's
 // tangential slopes), B (source of its time slope) and F (the flux).
-template <class T>
+template <bool FIRST, class T>
 __device__ __forceinline__ void side_block(const Gas<T>& g, const Maxw<T>& s, const NTab<T>& nh, const T DWx[5],
                                            const T DWy[5], const T DWz[5], T sx, T st, T c45, T c5, T p3, T p4,
                                            T p5, T by[5], T bz[5], T B[5], T F[5]) {
     const T h = s.h;
     const T rx = sx * s.ri, rt = st * s.ri;  // slope scale / rho
-    T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
-#pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWx[q] * rx;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, ax);
-#pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWy[q] * rt;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, ay);
-#pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWz[q] * rt;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, az);
+    Slv<T> S;
+    slv_make(g, s.U, s.lam, h, S);
+    T b[5], bp[5], ax[5], ay[5], az[5], sv[5];
+    to_frame(DWx, s.V, s.W, bp);
+    solve_scaled(S, rx, S.k4 * rx, S.l2 * rx, bp, ax);
+    const T k4t = S.k4 * rt, l2t = S.l2 * rt;
+    to_frame(DWy, s.V, s.W, bp);
+    solve_scaled(S, rt, k4t, l2t, bp, ay);
+    to_frame(DWz, s.V, s.W, bp);
+    solve_scaled(S, rt, k4t, l2t, bp, az);
 #pragma unroll
     for (int q = 0; q < 5; q++) sv[q] = s.V * ay[q] + s.W * az[q];
-    // time slope: A = -(<u psi ax> + <v psi ay> + <w psi az>) over the full space
+    GCoef<T> gc;
+    g_coef(ax, ay, az, sv, h, gc);
+    // time slope: A = -(<u psi ax> + <v psi ay> + <w psi az>) over the full
+    // space = -solve(G_0 of the full table); nA = -A
+    T nA[5];
     {
         NTab<T> nf;
         full_ntab(g, s.U, h, nf);
-#pragma unroll
-        for (int q = 0; q < 5; q++) b[q] = T(0);
-        acc_u<1>(b, T(-1), nf, h, ax);
-        acc_v<0>(b, T(-1), nf, h, ay);
-        acc_w<0>(b, T(-1), nf, h, az);
-        acc_u<0>(b, T(-1), nf, h, sv);
-        solve_frame(g, b, s.U, s.lam, h, A);
+        g_mom<0>(nf, gc, b);
+        solve_frame(S, b, nA);
     }
     T m[5];
     // equilibrium tangential-slope sources <psi ay>_half, <psi az>_half
-#pragma unroll
-    for (int q = 0; q < 5; q++) m[q] = T(0);
-    acc_u<0>(m, T(1), nh, h, ay);
-    add_to_lab(by, m, s.V, s.W);
-#pragma unroll
-    for (int q = 0; q < 5; q++) m[q] = T(0);
-    acc_u<0>(m, T(1), nh, h, az);
-    add_to_lab(bz, m, s.V, s.W);
+    u_mom<0>(nh, h, ay, m);
+    to_lab<!FIRST>(by, m, s.V, s.W);
+    u_mom<0>(nh, h, az, m);
+    to_lab<!FIRST>(bz, m, s.V, s.W);
     const T d = s.den;
     // time-slope source: (gm4+gm5) rho (<u psi ax> + <v psi ay> + <w psi az>) + gm5 rho <psi A>
-    T c[5];
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        m[q] = T(0);
-        c[q] = c45 * sv[q] + c5 * A[q];
-    }
-    const T dc45 = d * c45;
-    acc_u<1>(m, dc45, nh, h, ax);
-    acc_v<0>(m, dc45, nh, h, ay);
-    acc_w<0>(m, dc45, nh, h, az);
-    acc_u<0>(m, d, nh, h, c);
-    add_to_lab(B, m, s.V, s.W);
+    //   = rho (gm4+gm5) G_0 + rho gm5 u_0(A)
     // flux: p3 rho <u psi> + p4 rho (<u^2 psi ax> + <uv psi ay> + <uw psi az>) + p5 rho <u psi A>
+    //   = rho p4 G_1 + rho p5 u_1(A) + rho p3 (N_1, N_2, 0, 0, Q_1)
+    T G0[5], G1[5], uA0[5], uA1[5];
+    g_pair<0>(nh, gc, G0, G1);
+    u_pair<0>(nh, h, nA, uA0, uA1);
+    const T dc45 = d * c45, dc5 = d * c5, dp3 = d * p3, dp4 = d * p4, dp5 = d * p5;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        m[q] = T(0);
-        c[q] = p4 * sv[q] + p5 * A[q];
-    }
-    c[0] += p3;
-    const T dp4 = d * p4;
-    acc_u<2>(m, dp4, nh, h, ax);
-    acc_v<1>(m, dp4, nh, h, ay);
-    acc_w<1>(m, dp4, nh, h, az);
-    acc_u<1>(m, d, nh, h, c);
-    add_to_lab(F, m, s.V, s.W);
+    for (int q = 0; q < 5; q++) m[q] = dc45 * G0[q] - dc5 * uA0[q];
+    add_to_lab(B, m, s.V, s.W);
+#pragma unroll
+    for (int q = 0; q < 5; q++) m[q] = dp4 * G1[q] - dp5 * uA1[q];
+    m[0] += dp3 * nh.N[1];
+    m[1] += dp3 * nh.N[2];
+    m[4] += dp3 * nh.Q[1];
+    to_lab<!FIRST>(F, m, s.V, s.W);
 }
 
 // Time-integral coefficients of the BGK solution (GKMflux.cpp:155-157,
@@ -584,7 +681,7 @@ __device__ __forceinline__ void time_coef(float dt, float tau, TimeCoef<float>& 
 // (tangential): the scale factors are applied together with 1/rho.
 template <class T, class LoadSlopes>
 __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T WR[5], const T Wl[5],
-                                         const T Wr[5], LoadSlopes&& slopes, T sx, T st, T tau0, T dt, T dx,
+                                         const T Wr[5], LoadSlopes&& slopes, T sx, T st, T tau0, T dt, T idx,
                                          T F[5]) {
     Maxw<T> l, r;
     maxw_from_W(g, Wl, l);
@@ -607,56 +704,63 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     NTab<T> hl, hr;
     half_ntab(g, l.U, l.lam, l.h, +1, hl);
     half_ntab(g, r.U, r.lam, r.h, -1, hr);
-    T We[5] = {T(0), T(0), T(0), T(0), T(0)};
+    // We = rho_l <psi>_+ + rho_r <psi>_-, each side's moments rho (N_0, N_1, 0, 0, Q_0)
+    T We[5];
     {
-        const T e0[5] = {T(1), T(0), T(0), T(0), T(0)};
-        T m[5] = {T(0), T(0), T(0), T(0), T(0)};
-        acc_u<0>(m, l.den, hl, l.h, e0);
-        add_to_lab(We, m, l.V, l.W);
-#pragma unroll
-        for (int q = 0; q < 5; q++) m[q] = T(0);
-        acc_u<0>(m, r.den, hr, r.h, e0);
-        add_to_lab(We, m, r.V, r.W);
+        T m[5] = {l.den * hl.N[0], l.den * hl.N[1], T(0), T(0), l.den * hl.Q[0]};
+        const T vl = T(0.5) * (l.V * l.V + l.W * l.W);
+        We[0] = m[0];
+        We[1] = m[1];
+        We[2] = l.V * m[0];
+        We[3] = l.W * m[0];
+        We[4] = m[4] + vl * m[0];
+        m[0] = r.den * hr.N[0];
+        m[1] = r.den * hr.N[1];
+        m[4] = r.den * hr.Q[0];
+        const T vr = T(0.5) * (r.V * r.V + r.W * r.W);
+        We[0] += m[0];
+        We[1] += m[1];
+        We[2] += r.V * m[0];
+        We[3] += r.W * m[0];
+        We[4] += m[4] + vr * m[0];
     }
     Maxw<T> e;
     maxw_from_W(g, We, e);
 
     T by[5], bz[5], B[5], Fa[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        by[q] = T(0);
-        bz[q] = T(0);
-        B[q] = gm3 * We[q];
-        Fa[q] = T(0);
-    }
+    for (int q = 0; q < 5; q++) B[q] = gm3 * We[q];
     const T c45 = gm4 + gm5;
     {
         T Dx[5], Dy[5], Dz[5];
         slopes(0, Dx, Dy, Dz);  // left side
-        side_block(g, l, hl, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
+        side_block<true>(g, l, hl, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
     }
     {
         T Dx[5], Dy[5], Dz[5];
         slopes(1, Dx, Dy, Dz);  // right side
-        side_block(g, r, hr, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
+        side_block<false>(g, r, hr, Dx, Dy, Dz, sx, st, c45, gm5, p3, p4, p5, by, bz, B, Fa);
     }
 
     // equilibrium state, in its own co-moving frame
     const T he = e.h;
     T axl[5], axr[5], ay[5], az[5], A[5], b[5], bp[5], sv[5];
-    const T s2 = T(2) * e.ri * M<T>::rcp(dx);
+    Slv<T> S;
+    slv_make(g, e.U, e.lam, he, S);
+    const T s2 = T(2) * e.ri * idx;  // normal slopes over half a cell, / rho
+    const T k4x = S.k4 * s2, l2x = S.l2 * s2;
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = (We[q] - WL[q]) * s2;
+    for (int q = 0; q < 5; q++) b[q] = We[q] - WL[q];
     to_frame(b, e.V, e.W, bp);
-    solve_frame(g, bp, e.U, e.lam, he, axl);
+    solve_scaled(S, s2, k4x, l2x, bp, axl);
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = (WR[q] - We[q]) * s2;
+    for (int q = 0; q < 5; q++) b[q] = WR[q] - We[q];
     to_frame(b, e.V, e.W, bp);
-    solve_frame(g, bp, e.U, e.lam, he, axr);
+    solve_scaled(S, s2, k4x, l2x, bp, axr);
     to_frame(by, e.V, e.W, bp);
-    solve_frame(g, bp, e.U, e.lam, he, ay);
+    solve_frame(S, bp, ay);
     to_frame(bz, e.V, e.W, bp);
-    solve_frame(g, bp, e.U, e.lam, he, az);
+    solve_frame(S, bp, az);
 #pragma unroll
     for (int q = 0; q < 5; q++) sv[q] = e.V * ay[q] + e.W * az[q];
     const T de = e.den;
@@ -669,35 +773,34 @@ __device__ __forceinline__ void bgk_flux(const Gas<T>& g, const T WL[5], const T
     T dax[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) dax[q] = axl[q] - axr[q];
-    // time-slope source of the equilibrium part, added in the frame
+    // time-slope source of the equilibrium part, added in the frame:
+    //   rho gm2 (G_0(axr, ay, az, sv) + u_1(axl - axr)_+) + rho gm1 (N_0, N_1, 0, 0, Q_0)
+    GCoef<T> gc;
+    g_coef(axr, ay, az, sv, he, gc);
+    T G0[5], G1[5], ud1[5], ud2[5];
+    g_pair<0>(ef, gc, G0, G1);
+    u_pair<1>(ep, he, dax, ud1, ud2);
     to_frame(B, e.V, e.W, bp);
     {
-        const T dg2 = de * gm2;
-        T c[5];
+        const T dg1 = de * gm1, dg2 = de * gm2;
 #pragma unroll
-        for (int q = 0; q < 5; q++) c[q] = gm2 * sv[q];
-        c[0] += gm1;
-        acc_u<0>(bp, de, ef, he, c);
-        acc_u<1>(bp, dg2, ef, he, axr);
-        acc_u<1>(bp, dg2, ep, he, dax);
-        acc_v<0>(bp, dg2, ef, he, ay);
-        acc_w<0>(bp, dg2, ef, he, az);
+        for (int q = 0; q < 5; q++) bp[q] += dg2 * (G0[q] + ud1[q]);
+        bp[0] += dg1 * ef.N[0];
+        bp[1] += dg1 * ef.N[1];
+        bp[4] += dg1 * ef.Q[0];
     }
     const T ide = e.ri;
-#pragma unroll
-    for (int q = 0; q < 5; q++) bp[q] *= ide;
-    solve_frame(g, bp, e.U, e.lam, he, A);
+    solve_scaled(S, ide, S.k4 * ide, S.l2 * ide, bp, A);
     {
-        T m[5] = {T(0), T(0), T(0), T(0), T(0)}, c[5];
+        // flux: rho p2 (G_1 + u_2(axl - axr)_+) + rho p1 u_1(A) + rho p0 (N_1, N_2, 0, 0, Q_1)
+        T m[5], uA[5];
+        u_mom<1>(ef, he, A, uA);
+        const T dp0 = de * p0, dp1 = de * p1, dp2 = de * p2;
 #pragma unroll
-        for (int q = 0; q < 5; q++) c[q] = p1 * A[q] + p2 * sv[q];
-        c[0] += p0;
-        const T dp2 = de * p2;
-        acc_u<1>(m, de, ef, he, c);
-        acc_u<2>(m, dp2, ef, he, axr);
-        acc_u<2>(m, dp2, ep, he, dax);
-        acc_v<1>(m, dp2, ef, he, ay);
-        acc_w<1>(m, dp2, ef, he, az);
+        for (int q = 0; q < 5; q++) m[q] = dp2 * (G1[q] + ud2[q]) + dp1 * uA[q];
+        m[0] += dp0 * ef.N[1];
+        m[1] += dp0 * ef.N[2];
+        m[4] += dp0 * ef.Q[1];
         add_to_lab(Fa, m, e.V, e.W);
     }
 #pragma unroll
@@ -719,44 +822,33 @@ __device__ __forceinline__ void gks_flux(const Gas<T>& g, const T W0[5], const T
     const T h = s.h;
     NTab<T> nf;
     full_ntab(g, s.U, h, nf);
-    T b[5], bp[5], ax[5], ay[5], az[5], A[5], sv[5];
+    Slv<T> S;
+    slv_make(g, s.U, s.lam, h, S);
+    const T ri = s.ri, k4r = S.k4 * ri, l2r = S.l2 * ri;
+    T bp[5], ax[5], ay[5], az[5], nA[5], sv[5];
+    to_frame(DWx, s.V, s.W, bp);
+    solve_scaled(S, ri, k4r, l2r, bp, ax);
+    to_frame(DWy, s.V, s.W, bp);
+    solve_scaled(S, ri, k4r, l2r, bp, ay);
+    to_frame(DWz, s.V, s.W, bp);
+    solve_scaled(S, ri, k4r, l2r, bp, az);
 #pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWx[q] * s.ri;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, ax);
-#pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWy[q] * s.ri;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, ay);
-#pragma unroll
-    for (int q = 0; q < 5; q++) b[q] = DWz[q] * s.ri;
-    to_frame(b, s.V, s.W, bp);
-    solve_frame(g, bp, s.U, s.lam, h, az);
-#pragma unroll
-    for (int q = 0; q < 5; q++) {
-        sv[q] = s.V * ay[q] + s.W * az[q];
-        b[q] = T(0);
-    }
-    acc_u<1>(b, T(-1), nf, h, ax);
-    acc_v<0>(b, T(-1), nf, h, ay);
-    acc_w<0>(b, T(-1), nf, h, az);
-    acc_u<0>(b, T(-1), nf, h, sv);
-    solve_frame(g, b, s.U, s.lam, h, A);
+    for (int q = 0; q < 5; q++) sv[q] = s.V * ay[q] + s.W * az[q];
+    // A = -solve(G_0), nA = -A; F = rho (p1 G_1 + (p1 + p2) u_1(A) + p0 (N_1, N_2, 0, 0, Q_1))
+    GCoef<T> gc;
+    g_coef(ax, ay, az, sv, h, gc);
+    T G0[5], G1[5], uA[5], m[5];
+    g_pair<0>(nf, gc, G0, G1);
+    solve_frame(S, G0, nA);
+    u_mom<1>(nf, h, nA, uA);
     const T p0 = dt, p1 = -tau * dt, p2 = T(0.5) * dt * dt;
-    T c[5], m[5];
+    const T dp0 = s.den * p0, dp1 = s.den * p1, dp12 = s.den * (p1 + p2);
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        c[q] = p1 * sv[q] + (p1 + p2) * A[q];
-        m[q] = T(0);
-        F[q] = T(0);
-    }
-    c[0] += p0;
-    const T dp1 = s.den * p1;
-    acc_u<1>(m, s.den, nf, h, c);
-    acc_u<2>(m, dp1, nf, h, ax);
-    acc_v<1>(m, dp1, nf, h, ay);
-    acc_w<1>(m, dp1, nf, h, az);
-    add_to_lab(F, m, s.V, s.W);
+    for (int q = 0; q < 5; q++) m[q] = dp1 * G1[q] - dp12 * uA[q];
+    m[0] += dp0 * nf.N[1];
+    m[1] += dp0 * nf.N[2];
+    m[4] += dp0 * nf.Q[1];
+    to_lab<false>(F, m, s.V, s.W);
 }
 
 }  // namespace gkm
