@@ -774,10 +774,11 @@ int gkm_group_destroy(gkm_group_t* g) {
     return GKM_OK;
 }
 
-int gkm_set_timing(gkm_t* hh, int enabled) {
+int gkm_set_timing(gkm_t* hh, int mode) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");
-    h->timing = enabled != 0;
+    h->timing = mode != GKM_TIMING_OFF;
+    h->ev_mask = !h->timing ? 0u : mode == GKM_TIMING_FLUX ? gkm::kEvFluxXY : (1u << gkm::kEvPerStep) - 1u;
     return GKM_OK;
 }
 
@@ -789,6 +790,18 @@ int gkm_last_timing(gkm_t* hh, double* ms, int* nsteps) {
     if (!h->timing || h->timed_steps == 0) return GKM_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));
     double flux = 0, recon = 0, ghost = 0, upd = 0, prim = 0;
+    if (h->ev_mask == gkm::kEvFluxXY) {  // GKM_TIMING_FLUX: the X and Y flux kernels only
+        for (int s = 0; s < h->timed_steps; s++) {
+            hipEvent_t* ev = h->ev_step + s * gkm::kEvPerStep;
+            float t;
+            for (int d = 0; d < 2; d++) {
+                HIP_TRY(hipEventElapsedTime(&t, ev[2 + 3 * d], ev[3 + 3 * d]));
+                flux += t;
+            }
+        }
+        ms[0] = flux;
+        return GKM_OK;
+    }
     for (int s = 0; s < h->timed_steps; s++) {
         hipEvent_t* ev = h->ev_step + s * gkm::kEvPerStep;
         float t;
@@ -820,6 +833,7 @@ int gkm_last_sweep_timing(gkm_t* hh, double* ms) {
     if (!h->timing || h->timed_steps == 0) return GKM_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (int d = 0; d < 3; d++) {
+        if (!(h->ev_mask >> (3 + 3 * d) & 1u)) continue;  // not recorded in this mode
         double sum = 0;
         for (int s = 0; s < h->timed_steps; s++) {
             hipEvent_t* ev = h->ev_step + s * gkm::kEvPerStep;

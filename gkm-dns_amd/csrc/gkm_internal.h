@@ -14,6 +14,11 @@ constexpr int kTimedSteps = 64;  // steps per gkm_step call that carry HIP event
 // events of one step: 0 start, 12 after the ghost fill, 1 after the primitives,
 // 1+3d / 2+3d / 3+3d around sweep d's WENO and flux launches, 11 after the update
 constexpr int kEvPerStep = 13;
+// events of a step: 0 start, 12 after the ghost fill, 1 after the primitives;
+// sweep d (X, Y, Z) starts at 1 + 3 d: 2 + 3 d after its reconstruction, 3 + 3 d
+// after its flux (for Z the fused flux + update); 11 end.  GKM_TIMING_FLUX
+// records only 2, 3, 5, 6 (the X and Y flux kernels).
+constexpr unsigned kEvFluxXY = (1u << 2) | (1u << 3) | (1u << 5) | (1u << 6);
 
 struct Ctx {
     gkm_grid grid;
@@ -98,6 +103,7 @@ struct Ctx {
     void* halo_recv = nullptr;
     // timing
     bool timing = false;
+    unsigned ev_mask = 0;  // which of the kEvPerStep events of a timed step are recorded
     int timed_steps = 0;
     hipEvent_t ev_step[kTimedSteps * kEvPerStep];
     bool events_made = false;

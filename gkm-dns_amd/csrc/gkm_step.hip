@@ -1128,9 +1128,20 @@ static void launch_recon_any(Ctx* h, int dir, St st, const Real* Tw, Planes10<Re
     else launch_recon<Real, St, 0, 0>(h, dir, st, Tw, wi);
 }
 
+// Records event off + k of a timed step (base: the step's kEvPerStep events)
+// when the handle's timing mode asks for it; nothing when base is null.
+struct EvRec {
+    Ctx* h;
+    hipEvent_t* base;
+    int off;
+    void operator()(int k) const {
+        if (base && (h->ev_mask >> (off + k) & 1u)) (void)hipEventRecord(base[off + k], h->stream);
+    }
+};
+
 template <class Real, class St>
 static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau, Planes10<Real> wi,
-                         Planes5<Real> F, hipEvent_t* ev) {
+                         Planes5<Real> F, EvRec ev) {
     const Geo& g = h->geo;
     const int an = normal_axis(dir);
     const unsigned sn = axis_stride(g, an);
@@ -1140,9 +1151,9 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     Box3 bc = sweep_box(g, an, false);
     const int bs = h->flux_block;
     const dim3 grid(blocks_for(bc.count(), bs), h->nbox);
-    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    ev(0);
     if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction: one kernel per sweep
-        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        ev(1);
         if (dir == 0)
             hipLaunchKernelGGL((k_gks<Real, St, 0>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, F,
                                h->d_pars);
@@ -1152,12 +1163,12 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
         else
             hipLaunchKernelGGL((k_gks<Real, St, 2>), grid, dim3(bs), 0, h->stream, g, bc, sn, st1, st2, st, F,
                                h->d_pars);
-        if (ev) (void)hipEventRecord(ev[2], h->stream);
+        ev(2);
         return;
     }
     // WENO-BGK: the X and Y sweeps (the Z sweep's flux is fused with the update)
     launch_recon_any<Real, St>(h, dir, st, Tw, wi);
-    if (ev) (void)hipEventRecord(ev[1], h->stream);
+    ev(1);
     const size_t lds = sizeof(Real) == 8 ? 20 * sizeof(Real) * (size_t)bs : 0;  // flux_stage_slopes
     if (dir == 0)
         hipLaunchKernelGGL((k_flux<Real, St, 0>), grid, dim3(bs), lds, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
@@ -1165,7 +1176,7 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
     else
         hipLaunchKernelGGL((k_flux<Real, St, 1>), grid, dim3(bs), lds, h->stream, g, bc, sn, st1, st2, st, tau, wi, F,
                            h->d_pars);
-    if (ev) (void)hipEventRecord(ev[2], h->stream);
+    ev(2);
 }
 
 // Z sweep: reconstruction (WENO-BGK), then the flux fused with the update of
@@ -1174,7 +1185,7 @@ static void launch_sweep(Ctx* h, int dir, St st, const Real* Tw, const Real* tau
 // tau planes (Tw_nx, tau_nx), so the next step needs no ghost fill
 template <class Real, class St>
 static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Real* tau, Planes10<Real> wi,
-                                  Planes15<Real> F, hipEvent_t* ev, bool ghost, Real* Tw_nx, Real* tau_nx) {
+                                  Planes15<Real> F, EvRec ev, bool ghost, Real* Tw_nx, Real* tau_nx) {
     const Geo& g = h->geo;
     int t1, t2;
     tangential_axes(2, h->par.zbug_compat, t1, t2);
@@ -1184,26 +1195,26 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     const int bs = h->zflux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
     const size_t lds = (sizeof(Real) * UpdSlots<St>::n + sizeof(unsigned)) * (size_t)bs;
-    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    ev(0);
     auto fused = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(bs), lds, h->stream, g, zl, st1, st2, st, nx, tau, wi, F, h->d_pars,
                            Tw_nx, tau_nx, h->d_bad);
     };
     if (h->par.flux_scheme == GKM_SCHEME_GKS) {  // no reconstruction
-        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        ev(1);
         if (ghost)
             fused(k_flux_update<Real, St, GKM_SCHEME_GKS, true>);
         else
             fused(k_flux_update<Real, St, GKM_SCHEME_GKS, false>);
     } else {
         launch_recon_any<Real, St>(h, 2, st, Tw, wi);
-        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        ev(1);
         if (ghost)
             fused(k_flux_update<Real, St, GKM_SCHEME_WENO_BGK, true>);
         else
             fused(k_flux_update<Real, St, GKM_SCHEME_WENO_BGK, false>);
     }
-    if (ev) (void)hipEventRecord(ev[2], h->stream);
+    ev(2);
 }
 
 // after a step: the buffers swap, and the new current state's ghosts and
@@ -1312,7 +1323,8 @@ void ensure_ghosts(Ctx* h) {
     if (!h->derived_valid) launch_ghost(h, false);
 }
 
-void launch_step(Ctx* h, hipEvent_t* ev) {
+void launch_step(Ctx* h, hipEvent_t* evs) {
+    const EvRec ev{h, evs, 0};
     // the GKS scheme reads no per-cell primitive planes; both schemes fuse the
     // update into their Z sweep, which writes the other state buffer.  With
     // the periodic fill the Z sweep also writes that buffer's ghosts and
@@ -1321,14 +1333,14 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
     // step starts with the exchange.
     const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
     const bool pack = h->halo_mode == GKM_HALO_PACK;
-    if (ev) (void)hipEventRecord(ev[0], h->stream);
+    ev(0);
     if (pack)
         launch_ghost(h, prim);
     else
         prepare_step(h);
-    if (ev) (void)hipEventRecord(ev[12], h->stream);
+    ev(12);
     if (prim && pack) launch_prim(h);
-    if (ev) (void)hipEventRecord(ev[1], h->stream);
+    ev(1);
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st, nx;
         for (int q = 0; q < 5; q++) {
@@ -1342,9 +1354,9 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
         for (int dir = 0; dir < 2; dir++) {  // X and Y; Z is fused with the update
             Planes5<double> Fd;
             for (int q = 0; q < 5; q++) Fd.p[q] = h->Fd[5 * dir + q];
-            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, Fd, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<double, StateD>(h, dir, st, h->Td, h->taud, wi, Fd, EvRec{h, evs, 1 + 3 * dir});
         }
-        launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, ev ? ev + 7 : nullptr, !pack, h->Td,
+        launch_sweep_z_update<double, StateD>(h, st, nx, h->Td, h->taud, wi, F, EvRec{h, evs, 7}, !pack, h->Td,
                                               h->taud_nx);
     } else {
         StateM st, nx;
@@ -1361,12 +1373,12 @@ void launch_step(Ctx* h, hipEvent_t* ev) {
         for (int dir = 0; dir < 2; dir++) {
             Planes5<float> Ff;
             for (int q = 0; q < 5; q++) Ff.p[q] = h->Ff[5 * dir + q];
-            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, Ff, ev ? ev + 1 + 3 * dir : nullptr);
+            launch_sweep<float, StateM>(h, dir, st, h->Tf, h->tauf, wi, Ff, EvRec{h, evs, 1 + 3 * dir});
         }
-        launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, ev ? ev + 7 : nullptr, !pack, h->Tf,
+        launch_sweep_z_update<float, StateM>(h, st, nx, h->Tf, h->tauf, wi, F, EvRec{h, evs, 7}, !pack, h->Tf,
                                              h->tauf_nx);
     }
-    if (ev) (void)hipEventRecord(ev[11], h->stream);
+    ev(11);
     swap_state(h);
 }
 

@@ -301,8 +301,12 @@ class Solver:
         _abi.check(self._L.gkm_set_halo_mode(self._h, _abi.HALO[mode]))
 
     # -- timing -----------------------------------------------------------------------
-    def set_timing(self, on: bool = True) -> None:
-        _abi.check(self._L.gkm_set_timing(self._h, int(on)))
+    def set_timing(self, on: bool | str = True) -> None:
+        """Device timing of the following steps: True or "all" (an event around
+        every launch), "flux" (events around the X and Y flux kernels only, the
+        least perturbing), False or "off"."""
+        mode = {"off": 0, "all": 1, "flux": 2}[on] if isinstance(on, str) else int(bool(on))
+        _abi.check(self._L.gkm_set_timing(self._h, mode))
 
     def last_timing(self):
         ms = (C.c_double * 5)()

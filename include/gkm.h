@@ -46,6 +46,10 @@ extern "C" {
                                   buffer (the single-GPU stand-in for a slab exchange), then the
                                   x/z ghosts filled in place                                      */
 
+#define GKM_TIMING_OFF  0      /* gkm_set_timing: no events                                     */
+#define GKM_TIMING_ALL  1      /* an event around every launch of the step                     */
+#define GKM_TIMING_FLUX 2      /* events around the X and Y flux kernels only (least perturbing) */
+
 #define GKM_PREC_FP64  0       /* FP64 everywhere (drop-in form)                               */
 #define GKM_PREC_MIXED 1       /* FP32 reconstruction + flux, FP32 hi/lo compensated state     */
 
@@ -281,8 +285,13 @@ int gkm_sync(gkm_t* h);
  * fill (in GKM_HALO_GATHER mode fused with the primitives), ms[3]
  * conservative updates, ms[4] the separate primitives kernel of
  * GKM_HALO_PACK mode (0 otherwise); -1 where nothing was recorded.
- * *nsteps receives the number of timed steps.  Timing is off by default. */
-int gkm_set_timing(gkm_t* h, int enabled);
+ * *nsteps receives the number of timed steps.  Timing is off by default.
+ * mode: GKM_TIMING_ALL (any other nonzero value too) records every event;
+ * GKM_TIMING_FLUX records only those around the X and Y flux kernels, so the
+ * step runs with four events instead of thirteen: ms[0] is then those two
+ * kernels' time and ms[1..4] are -1 (gkm_last_sweep_timing gives X and Y, Z
+ * -1). */
+int gkm_set_timing(gkm_t* h, int mode);
 int gkm_last_timing(gkm_t* h, double ms[5], int* nsteps);
 /* Flux-kernel time of each sweep direction (X, Y, Z) over the timed steps of
  * the last gkm_step call; the Z entry is the flux kernel that also applies
