@@ -490,9 +490,11 @@ __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAx
         }
     };
     auto put_f = [&](int l, int n, const double (&v)[4]) {
-        tile[(0 * m + n) * kSplL + l] = v[1] / v[0];
-        tile[(1 * m + n) * kSplL + l] = v[2] / v[0];
-        tile[(2 * m + n) * kSplL + l] = v[3] / v[0];
+        // one FP64 division instead of three (PrintStats 1.111 -> 1.102 ms at 256^3)
+        const double iv = 1.0 / v[0];
+        tile[(0 * m + n) * kSplL + l] = v[1] * iv;
+        tile[(1 * m + n) * kSplL + l] = v[2] * iv;
+        tile[(2 * m + n) * kSplL + l] = v[3] * iv;
     };
     {
         static_assert(A == 2, "x and y lines go through k_spl_stream");
