@@ -65,69 +65,17 @@ template <> struct M<float> {
 // interface).  SYMOO: weno/src/update.cpp:507-557, GPUupdate.cu:917-977.
 // JS: update.cpp:1421-1468.
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ void weno_symoo(T LLL, T LL, T L, T R, T RR, T RRR, T eps, T& wl, T& wr) {
-    const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
-    const T c1312 = T(13) / T(12);
-    T P0p = LLL * c13 - LL * c76 + L * c116;
-    T P0n = RRR * c13 - RR * c76 + R * c116;
-    T P1p = -LL * c16 + L * c56 + R * c13;
-    T P1n = -RR * c16 + R * c56 + L * c13;
-    T P2p = L * c13 + R * c56 - RR * c16;
-    T P2n = R * c13 + L * c56 - LL * c16;
-    T P3p = R * c116 - RR * c76 + RRR * c13;
-    T P3n = L * c116 - LL * c76 + LLL * c13;
-    T a, b;
-    a = LLL - 2 * LL + L; b = LLL - 4 * LL + 3 * L; T IS0p = c1312 * a * a + T(0.25) * b * b;
-    a = RRR - 2 * RR + R; b = RRR - 4 * RR + 3 * R; T IS0n = c1312 * a * a + T(0.25) * b * b;
-    a = LL - 2 * L + R;   b = R - LL;               T IS1p = c1312 * a * a + T(0.25) * b * b;
-    a = RR - 2 * R + L;   b = L - RR;               T IS1n = c1312 * a * a + T(0.25) * b * b;
-    a = L - 2 * R + RR;   b = -3 * L + 4 * R - RR;  T IS2p = c1312 * a * a + T(0.25) * b * b;
-    a = R - 2 * L + LL;   b = -3 * R + 4 * L - LL;  T IS2n = c1312 * a * a + T(0.25) * b * b;
-    a = R - 2 * RR + RRR; b = -5 * R + 8 * RR - 3 * RRR; T IS3p = c1312 * a * a + T(0.25) * b * b;
-    a = L - 2 * LL + LLL; b = -5 * L + 8 * LL - 3 * LLL; T IS3n = c1312 * a * a + T(0.25) * b * b;
-    // the downwind indicator is replaced by the largest of the four
-    IS3p = fmax(fmax(fmax(IS0p, IS1p), IS2p), IS3p);
-    IS3n = fmax(fmax(fmax(IS0n, IS1n), IS2n), IS3n);
-    const T g0 = T(1) / T(20), g1 = T(9) / T(20);
-    T A0 = g0 * M<T>::rcp(eps + IS0p), A1 = g1 * M<T>::rcp(eps + IS1p);
-    T A2 = g1 * M<T>::rcp(eps + IS2p), A3 = g0 * M<T>::rcp(eps + IS3p);
-    wl = (A0 * P0p + A1 * P1p + A2 * P2p + A3 * P3p) * M<T>::rcp(A0 + A1 + A2 + A3);
-    A0 = g0 * M<T>::rcp(eps + IS0n); A1 = g1 * M<T>::rcp(eps + IS1n);
-    A2 = g1 * M<T>::rcp(eps + IS2n); A3 = g0 * M<T>::rcp(eps + IS3n);
-    wr = (A0 * P0n + A1 * P1n + A2 * P2n + A3 * P3n) * M<T>::rcp(A0 + A1 + A2 + A3);
-}
-
-template <class T>
-__device__ __forceinline__ void weno_js(T LLL, T LL, T L, T R, T RR, T RRR, T eps, T& wl, T& wr) {
-    const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
-    const T c1312 = T(13) / T(12);
-    T P0p = LLL * c13 - LL * c76 + L * c116;
-    T P0n = R * c116 - RR * c76 + RRR * c13;
-    T P1p = -LL * c16 + L * c56 + R * c13;
-    T P1n = L * c13 + R * c56 - RR * c16;
-    T P2p = P1n;  // L/3 + 5R/6 - RR/6
-    T P2n = P1p;  // -LL/6 + 5L/6 + R/3
-    T a, b;
-    a = LLL - 2 * LL + L; b = LLL - 4 * LL + 3 * L; T IS0p = c1312 * a * a + T(0.25) * b * b;
-    a = RRR - 2 * RR + R; b = RRR - 4 * RR + 3 * R; T IS0n = c1312 * a * a + T(0.25) * b * b;
-    a = LL - 2 * L + R;   b = LL - R;               T IS1p = c1312 * a * a + T(0.25) * b * b;
-    a = L - 2 * R + RR;   b = L - RR;               T IS1n = c1312 * a * a + T(0.25) * b * b;
-    b = 3 * L - 4 * R + RR;                         T IS2p = c1312 * a * a + T(0.25) * b * b;
-    a = LL - 2 * L + R;   b = LL - 4 * L + 3 * R;   T IS2n = c1312 * a * a + T(0.25) * b * b;
-    T r;
-    r = M<T>::rcp(eps + IS0p); T A0 = T(0.1) * r * r;
-    r = M<T>::rcp(eps + IS1p); T A1 = T(0.6) * r * r;
-    r = M<T>::rcp(eps + IS2p); T A2 = T(0.3) * r * r;
-    wl = (A0 * P0p + A1 * P1p + A2 * P2p) * M<T>::rcp(A0 + A1 + A2);
-    r = M<T>::rcp(eps + IS0n); A0 = T(0.1) * r * r;
-    r = M<T>::rcp(eps + IS1n); A1 = T(0.6) * r * r;
-    r = M<T>::rcp(eps + IS2n); A2 = T(0.3) * r * r;
-    wr = (A0 * P0n + A1 * P1n + A2 * P2n) * M<T>::rcp(A0 + A1 + A2);
-}
-
-// The same two schemes for a thread that walks consecutive interfaces along a
-// line.  The stencils of the right value at interface i+1/2 are those of the
+// Floating-point contraction in the WENO code is statement-local: a multiply
+// is fused into an add only within one expression, never across statements.
+// Fusing across statements (the compiler's default for HIP) makes the choice
+// depend on the code around an inlined call -- how often a product is used --
+// so the same indicator formed in two kernels could round differently.  With
+// it statement-local, an indicator or weight has the same bits in every
+// kernel that forms it, which the carried weights and the Z sweep's lane
+// exchange rely on.
+#pragma clang fp contract(on)
+// Both schemes are written for a thread that walks consecutive interfaces
+// along a line.  The stencils of the right value at interface i+1/2 are those of the
 // left value at i+3/2 (the cells i-1..i+3 shifted by one), so their
 // smoothness weights are carried instead of recomputed:
 //   JS:    1/(eps+IS)^2 of IS2n, IS1n, IS0n -> IS0p, IS1p, IS2p of the next interface
@@ -153,25 +101,40 @@ template <class T> __device__ __forceinline__ void js_pset(T LLL, T LL, T L, T R
     r = M<T>::rcp(eps + is_sq(RR - 2 * R + L, RR - 4 * R + 3 * L));   c.r[2] = r * r;
 }
 
+// right-side weights of interface i+1/2, stored as the left-side weights of
+// interface i+3/2 want them (IS2n, IS1n, IS0n -> IS0p, IS1p, IS2p)
+template <class T> __device__ __forceinline__ void js_right(T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c) {
+    T r0, r1, r2;
+    r0 = M<T>::rcp(eps + is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R)); r0 = r0 * r0;
+    r1 = M<T>::rcp(eps + is_sq(L - 2 * R + RR, L - RR));                 r1 = r1 * r1;
+    r2 = M<T>::rcp(eps + is_sq(LL - 2 * L + R, LL - 4 * L + 3 * R));     r2 = r2 * r2;
+    c.r[0] = r2;
+    c.r[1] = r1;
+    c.r[2] = r0;
+}
+
+// both values of interface i+1/2 from its left-side weights (lw) and right-side ones (rw, as js_right stores them)
 template <class T>
-__device__ __forceinline__ void weno_js_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
-                                              T& wr) {
+__device__ __forceinline__ void weno_js_lr(T LLL, T LL, T L, T R, T RR, T RRR, const Carry<T>& lw, const Carry<T>& rw,
+                                           T& wl, T& wr) {
     const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
     const T P0p = LLL * c13 - LL * c76 + L * c116;
     const T P0n = R * c116 - RR * c76 + RRR * c13;
     const T P1p = -LL * c16 + L * c56 + R * c13;
     const T P1n = L * c13 + R * c56 - RR * c16;
-    T A0 = T(0.1) * c.r[0], A1 = T(0.6) * c.r[1], A2 = T(0.3) * c.r[2];
+    T A0 = T(0.1) * lw.r[0], A1 = T(0.6) * lw.r[1], A2 = T(0.3) * lw.r[2];
     wl = (A0 * P0p + A1 * P1p + A2 * P1n) * M<T>::rcp(A0 + A1 + A2);
-    T r0, r1, r2;
-    r0 = M<T>::rcp(eps + is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R)); r0 = r0 * r0;
-    r1 = M<T>::rcp(eps + is_sq(L - 2 * R + RR, L - RR));                 r1 = r1 * r1;
-    r2 = M<T>::rcp(eps + is_sq(LL - 2 * L + R, LL - 4 * L + 3 * R));     r2 = r2 * r2;
-    A0 = T(0.1) * r0; A1 = T(0.6) * r1; A2 = T(0.3) * r2;
+    A0 = T(0.1) * rw.r[2]; A1 = T(0.6) * rw.r[1]; A2 = T(0.3) * rw.r[0];
     wr = (A0 * P0n + A1 * P1n + A2 * P1p) * M<T>::rcp(A0 + A1 + A2);
-    c.r[0] = r2;  // IS2n -> IS0p of the next interface
-    c.r[1] = r1;
-    c.r[2] = r0;
+}
+
+template <class T>
+__device__ __forceinline__ void weno_js_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
+                                              T& wr) {
+    Carry<T> rw;
+    js_right(LL, L, R, RR, RRR, eps, rw);
+    weno_js_lr(LLL, LL, L, R, RR, RRR, c, rw, wl, wr);
+    c = rw;
 }
 
 template <class T> __device__ __forceinline__ void symoo_pset(T LLL, T LL, T L, T R, T RR, T eps, Carry<T>& c) {
@@ -182,9 +145,22 @@ template <class T> __device__ __forceinline__ void symoo_pset(T LLL, T LL, T L, 
     for (int k = 0; k < 3; k++) c.r[k] = M<T>::rcp(eps + c.is[k]);
 }
 
+// right-side indicators and weights of interface i+1/2, stored as interface
+// i+3/2's left side wants them (IS2n, IS1n, IS0n -> IS0p, IS1p, IS2p)
+template <class T> __device__ __forceinline__ void symoo_right(T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c) {
+    const T IS0n = is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R);
+    const T IS1n = is_sq(RR - 2 * R + L, L - RR);
+    const T IS2n = is_sq(R - 2 * L + LL, -3 * R + 4 * L - LL);
+    c.is[0] = IS2n; c.r[0] = M<T>::rcp(eps + IS2n);
+    c.is[1] = IS1n; c.r[1] = M<T>::rcp(eps + IS1n);
+    c.is[2] = IS0n; c.r[2] = M<T>::rcp(eps + IS0n);
+}
+
+// both values of interface i+1/2 from its left-side (lw) and right-side (rw) indicators and weights; the
+// fourth stencil of each side is new cells and formed here
 template <class T>
-__device__ __forceinline__ void weno_symoo_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
-                                                 T& wr) {
+__device__ __forceinline__ void weno_symoo_lr(T LLL, T LL, T L, T R, T RR, T RRR, T eps, const Carry<T>& lw,
+                                              const Carry<T>& rw, T& wl, T& wr) {
     const T c13 = T(1) / T(3), c56 = T(5) / T(6), c16 = T(1) / T(6), c76 = T(7) / T(6), c116 = T(11) / T(6);
     const T P0p = LLL * c13 - LL * c76 + L * c116;
     const T P0n = RRR * c13 - RR * c76 + R * c116;
@@ -197,20 +173,41 @@ __device__ __forceinline__ void weno_symoo_carry(T LLL, T LL, T L, T R, T RR, T 
     const T g0 = T(1) / T(20), g1 = T(9) / T(20);
     // left value: IS0p..IS2p carried, IS3p from the new stencil, replaced by the max
     T IS3p = is_sq(R - 2 * RR + RRR, -5 * R + 8 * RR - 3 * RRR);
-    IS3p = fmax(fmax(fmax(c.is[0], c.is[1]), c.is[2]), IS3p);
-    T A0 = g0 * c.r[0], A1 = g1 * c.r[1], A2 = g1 * c.r[2], A3 = g0 * M<T>::rcp(eps + IS3p);
+    IS3p = fmax(fmax(fmax(lw.is[0], lw.is[1]), lw.is[2]), IS3p);
+    T A0 = g0 * lw.r[0], A1 = g1 * lw.r[1], A2 = g1 * lw.r[2], A3 = g0 * M<T>::rcp(eps + IS3p);
     wl = (A0 * P0p + A1 * P1p + A2 * P2p + A3 * P3p) * M<T>::rcp(A0 + A1 + A2 + A3);
-    const T IS0n = is_sq(RRR - 2 * RR + R, RRR - 4 * RR + 3 * R);
-    const T IS1n = is_sq(RR - 2 * R + L, L - RR);
-    const T IS2n = is_sq(R - 2 * L + LL, -3 * R + 4 * L - LL);
     T IS3n = is_sq(L - 2 * LL + LLL, -5 * L + 8 * LL - 3 * LLL);
-    IS3n = fmax(fmax(fmax(IS0n, IS1n), IS2n), IS3n);
-    const T r0 = M<T>::rcp(eps + IS0n), r1 = M<T>::rcp(eps + IS1n), r2 = M<T>::rcp(eps + IS2n);
-    A0 = g0 * r0; A1 = g1 * r1; A2 = g1 * r2; A3 = g0 * M<T>::rcp(eps + IS3n);
+    IS3n = fmax(fmax(fmax(rw.is[2], rw.is[1]), rw.is[0]), IS3n);
+    A0 = g0 * rw.r[2]; A1 = g1 * rw.r[1]; A2 = g1 * rw.r[0]; A3 = g0 * M<T>::rcp(eps + IS3n);
     wr = (A0 * P0n + A1 * P1n + A2 * P2n + A3 * P3n) * M<T>::rcp(A0 + A1 + A2 + A3);
-    c.is[0] = IS2n; c.r[0] = r2;  // stencil i-1..i+1 -> IS0p of the next interface
-    c.is[1] = IS1n; c.r[1] = r1;
-    c.is[2] = IS0n; c.r[2] = r0;
+}
+
+template <class T>
+__device__ __forceinline__ void weno_symoo_carry(T LLL, T LL, T L, T R, T RR, T RRR, T eps, Carry<T>& c, T& wl,
+                                                 T& wr) {
+    Carry<T> rw;
+    symoo_right(LL, L, R, RR, RRR, eps, rw);
+    weno_symoo_lr(LLL, LL, L, R, RR, RRR, eps, c, rw, wl, wr);
+    c = rw;
+}
+
+// Single interface: the first interface's left-side weights, then the
+// carried form.  Every path (the strided windows, the Z sweep's lanes, the
+// persistent kernel, the device arithmetic checks) therefore forms each
+// indicator and weight with the same expression, whichever interface supplied
+// it, and gets the same bits.
+template <class T>
+__device__ __forceinline__ void weno_symoo(T LLL, T LL, T L, T R, T RR, T RRR, T eps, T& wl, T& wr) {
+    Carry<T> c;
+    symoo_pset(LLL, LL, L, R, RR, eps, c);
+    weno_symoo_carry(LLL, LL, L, R, RR, RRR, eps, c, wl, wr);
+}
+
+template <class T>
+__device__ __forceinline__ void weno_js(T LLL, T LL, T L, T R, T RR, T RRR, T eps, T& wl, T& wr) {
+    Carry<T> c;
+    js_pset(LLL, LL, L, R, RR, eps, c);
+    weno_js_carry(LLL, LL, L, R, RR, RRR, eps, c, wl, wr);
 }
 
 template <int RECON, class T>
@@ -228,6 +225,22 @@ template <int RECON, class T> __device__ __forceinline__ void weno_pset(const T 
         symoo_pset(c[0], c[1], c[2], c[3], c[4], eps, k);
 }
 
+// the right-side weights of a line's interface, and both values from the left- and right-side ones
+template <int RECON, class T> __device__ __forceinline__ void weno_right(const T c[6], T eps, Carry<T>& k) {
+    if (RECON == 1)
+        js_right(c[1], c[2], c[3], c[4], c[5], eps, k);
+    else
+        symoo_right(c[1], c[2], c[3], c[4], c[5], eps, k);
+}
+template <int RECON, class T>
+__device__ __forceinline__ void weno_line_lr(const T c[6], T eps, const Carry<T>& lw, const Carry<T>& rw, T& wl,
+                                             T& wr) {
+    if (RECON == 1)
+        weno_js_lr(c[0], c[1], c[2], c[3], c[4], c[5], lw, rw, wl, wr);
+    else
+        weno_symoo_lr(c[0], c[1], c[2], c[3], c[4], c[5], eps, lw, rw, wl, wr);
+}
+
 template <int RECON, class T>
 __device__ __forceinline__ void weno_line(const T c[6], T eps, T& wl, T& wr) {
     if (RECON == 1)
@@ -235,6 +248,8 @@ __device__ __forceinline__ void weno_line(const T c[6], T eps, T& wl, T& wr) {
     else
         weno_symoo(c[0], c[1], c[2], c[3], c[4], c[5], eps, wl, wr);
 }
+
+#pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
 // BGK-NS flux

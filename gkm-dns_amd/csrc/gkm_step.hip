@@ -155,6 +155,9 @@ __global__ void __launch_bounds__(256) k_prim(Geo g, St st, Real* Tw, Real* tau,
 // Temperature mode (update.cpp:560-621): rho, rho u, rho v, rho w by WENO, the
 // temperature (from the per-cell plane) by WENO, rho E rebuilt from both.
 // ---------------------------------------------------------------------------
+// (the reconstruction kernels fuse multiplies into adds only within a statement,
+// as gkm_device.h's WENO code does, so that every kernel forms the same bits)
+#pragma clang fp contract(on)
 template <int RECON, int ENERGY, class Real>
 __device__ __forceinline__ void recon_one(const Real (&v)[6][6], Real eps, const StepPar& par, Real wl[5],
                                           Real wr[5]) {
@@ -187,8 +190,10 @@ __device__ __forceinline__ void recon_one_carry(const Real (&v)[6][6], Real eps,
     }
 }
 
-// Unit-stride sweep (Z): one interface per thread; neighbouring threads share
-// their six-cell stencils through the cache.
+// Unit-stride sweep (Z) as the persistent group kernel runs it: one interface
+// per thread, the six-cell stencils from memory (neighbouring threads share
+// them through the cache).  The per-stage Z sweep is k_recon_zc below, which
+// gives the same bits.
 // (the block functions below take the block's place -- bx along the work, box
 // -- as arguments, so that the persistent group kernel can run them too)
 template <class Real, class St, int RECON, int ENERGY>
@@ -216,65 +221,135 @@ __device__ __forceinline__ void recon_z_block(const Geo& g, const Box3& b, unsig
     }
 }
 
-template <class Real, class St, int RECON, int ENERGY>
-__global__ void __launch_bounds__(256) k_recon(Geo g, Box3 b, unsigned sn, St st, const Real* __restrict__ Tw,
-                                               Planes10<Real> wi, const StepPar* __restrict__ pars) {
-    recon_z_block<Real, St, RECON, ENERGY>(g, b, sn, st, Tw, wi, pars[blockIdx.y], blockIdx.x, blockIdx.y);
+// The Z sweep with the smoothness weights carried across lanes and an LDS
+// row tile.
+//
+// Positions: each (j, i) line of the extended set has NP = nI + 1 positions
+// p = 0 .. nI for the padded interfaces k = p + 1; p = 0 (k = 1) is a
+// virtual position whose only use is its right-side weights, which are the
+// left-side ones of the line's first interface (k = 2).  Positions are
+// numbered line by line, k fastest, and wavefront w takes the 64 positions
+// from 63 w, so consecutive wavefronts share one.  Every lane forms the
+// right-side weights of its interface (weno_right); lane l takes its
+// left-side ones from lane l - 1 (the previous interface of its line, or the
+// line's virtual position) and forms both values (weno_line_lr).  Lane 0,
+// whose position the previous wavefront's lane 63 writes, and the virtual
+// positions write nothing, so every interface is written exactly once, with
+// the bits of recon_z_block (the single-interface form forms every weight
+// with the same expression).
+//
+// Tile: a block's 253 positions lie on consecutive z lines of one j row,
+// which are contiguous in memory, so the cells its stencils read form one
+// range [c_first - 2, c_last + 3].  The block loads that range once
+// (coalesced, each cell once) and reads the six-cell stencils from LDS.  A
+// line end skips 4 cells and 252 positions cross at most ceil(252 / NP) of
+// them, so the range is at most span_max = 258 + 4 ceil(252 / NP) cells; a
+// block whose range is longer (it spans two j rows) loads from memory.
+struct ZPos {
+    unsigned count;  // positions of one box: lines * NP
+    FastDiv fNP;     // NP = nI + 1 positions per line
+    FastDiv fn1;     // lines per j row (the extended i range)
+};
+
+constexpr unsigned kZPosBlock = 4u * 63u;  // new positions per 256-thread block
+
+__device__ __forceinline__ unsigned zpos_cell(const Geo& g, const ZPos& zp, unsigned q, unsigned& p) {
+    const unsigned line = fdiv(q, zp.fNP);
+    p = q - line * zp.fNP.d;
+    const unsigned j = fdiv(line, zp.fn1);
+    const unsigned i = line - j * zp.fn1.d;
+    return (j + 2) * g.s0 + (i + 2) * g.s1 + (p + 1);
 }
 
-// The Z sweep with an LDS row tile.  A block's consecutive interfaces lie on
-// consecutive z lines of one j row, which are contiguous in memory, so the
-// cells its stencils read form one range [c_first - 2, c_last + 3]: the block
-// loads that range once (coalesced, each cell once) and reads the six-cell
-// stencils from LDS.  Inside a row, 256 consecutive interfaces cross at most
-// ceil(255 / nI) line ends and each crossing skips 5 cells, so the range is at
-// most span_max = 261 + 5 ceil(255 / nI) cells; a block whose range is longer
-// (it spans two j rows) loads from memory like k_recon.
+template <class T> __device__ __forceinline__ Carry<T> carry_from_prev_lane(const Carry<T>& c, bool symoo) {
+    Carry<T> o;
+#pragma unroll
+    for (int k = 0; k < 3; k++) o.r[k] = __shfl_up(c.r[k], 1u, 64);
+    if (symoo) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) o.is[k] = __shfl_up(c.is[k], 1u, 64);
+    }
+    return o;
+}
+
 template <class Real, class St, int RECON, int ENERGY>
-__global__ void __launch_bounds__(256) k_recon_zt(Geo g, Box3 b, St st, const Real* __restrict__ Tw,
+__global__ void __launch_bounds__(256) k_recon_zc(Geo g, ZPos zp, St st, const Real* __restrict__ Tw,
                                                   Planes10<Real> wi, const StepPar* __restrict__ pars,
                                                   unsigned span_max) {
     extern __shared__ unsigned char zt_raw[];
     Real* tile = reinterpret_cast<Real*>(zt_raw);  // [6][span_max]: fields 0..4 (or 0..3 and T in slot 5)
     constexpr int nlin = ENERGY == 1 ? 5 : 4;
-    const unsigned n = b.count();
-    const unsigned t0 = blockIdx.x * blockDim.x, t = t0 + threadIdx.x;
     const unsigned box0 = blockIdx.y * g.ncell;
-    const unsigned lo = box_index(g, b, t0) - 2u;
-    const unsigned span = box_index(g, b, min(t0 + blockDim.x, n) - 1u) + 3u - lo + 1u;
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned q0 = blockIdx.x * kZPosBlock;
+    const unsigned q = q0 + (threadIdx.x >> 6) * 63u + lane;
+    const bool valid = q < zp.count;
+    unsigned pf, pl, p;
+    const unsigned lo = zpos_cell(g, zp, q0, pf) - 2u;
+    const unsigned span = zpos_cell(g, zp, min(q0 + kZPosBlock, zp.count - 1u), pl) + 3u - lo + 1u;
     const bool tiled = span <= span_max;  // block-uniform
     if (tiled) {
         for (unsigned z = threadIdx.x; z < span; z += blockDim.x) {
             const unsigned cc = box0 + lo + z;
 #pragma unroll
-            for (int q = 0; q < nlin; q++) tile[q * span_max + z] = (Real)st.cell(q, cc);
+            for (int f = 0; f < nlin; f++) tile[f * span_max + z] = (Real)st.cell(f, cc);
             if (ENERGY == 0) tile[5 * span_max + z] = ld(Tw, cc);
         }
         __syncthreads();
     }
-    if (t >= n) return;
-    if (!tiled) {
-        recon_z_block<Real, St, RECON, ENERGY>(g, b, 1u, st, Tw, wi, pars[blockIdx.y], blockIdx.x, blockIdx.y);
-        return;
+    // lanes past the end take the last position (in bounds, never written) so
+    // that the whole wavefront reaches the lane exchange
+    const unsigned cl = zpos_cell(g, zp, valid ? q : zp.count - 1u, p);
+    Real v[6][6];
+    if (tiled) {
+        const unsigned o = cl - lo - 2u;  // the stencil's first cell in the tile
+#pragma unroll
+        for (int f = 0; f < nlin; f++)
+#pragma unroll
+            for (int s = 0; s < 6; s++) v[f][s] = tile[f * span_max + o + s];
+        if (ENERGY == 0)
+#pragma unroll
+            for (int s = 0; s < 6; s++) v[5][s] = tile[5 * span_max + o + s];
+    } else {
+        const unsigned c0 = box0 + cl - 2u;
+#pragma unroll
+        for (int f = 0; f < nlin; f++)
+#pragma unroll
+            for (int s = 0; s < 6; s++) v[f][s] = (Real)st.cell(f, c0 + s);
+        if (ENERGY == 0)
+#pragma unroll
+            for (int s = 0; s < 6; s++) v[5][s] = ld(Tw, c0 + s);
     }
     const StepPar& par = pars[blockIdx.y];
-    const unsigned cl = box_index(g, b, t);
-    const unsigned o = cl - lo - 2u;  // the stencil's first cell in the tile
-    Real v[6][6];
+    const Real eps = Real(1e-10);
+    Real wl[5], wr[5], Tl = Real(0), Tr = Real(0);
 #pragma unroll
-    for (int q = 0; q < nlin; q++)
-#pragma unroll
-        for (int s = 0; s < 6; s++) v[q][s] = tile[q * span_max + o + s];
-    if (ENERGY == 0)
-#pragma unroll
-        for (int s = 0; s < 6; s++) v[5][s] = tile[5 * span_max + o + s];
-    Real wl[5], wr[5];
-    recon_one<RECON, ENERGY>(v, Real(1e-10), par, wl, wr);
+    for (int f = 0; f < 6; f++) {
+        if (f >= nlin && !(ENERGY == 0 && f == 5)) continue;
+        Carry<Real> rw;
+        weno_right<RECON>(v[f], eps, rw);
+        const Carry<Real> lw = carry_from_prev_lane(rw, RECON != 1);
+        Real a, b;
+        weno_line_lr<RECON>(v[f], eps, lw, rw, a, b);
+        if (f < 5) {
+            wl[f] = a;
+            wr[f] = b;
+        } else {
+            Tl = a;
+            Tr = b;
+        }
+    }
+    if (ENERGY == 0) {
+        const Real cR = upar<Real>(par, kUCR);
+        wl[4] = wl[0] * cR * Tl + Real(0.5) * (wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]) * M<Real>::rcp(wl[0]);
+        wr[4] = wr[0] * cR * Tr + Real(0.5) * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]) * M<Real>::rcp(wr[0]);
+    }
+    if (!valid || lane == 0u || p == 0u) return;
     const unsigned c = box0 + cl;
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        put(wi.p[q], c, wl[q]);
-        put(wi.p[5 + q], c, wr[q]);
+    for (int f = 0; f < 5; f++) {
+        put(wi.p[f], c, wl[f]);
+        put(wi.p[5 + f], c, wr[f]);
     }
 }
 
@@ -356,6 +431,8 @@ __global__ void __launch_bounds__(256) k_recon_win(Geo g, Box3 b, unsigned sn, u
     recon_win_block<Real, St, RECON, ENERGY, CHUNK>(g, b, sn, sa, n_normal, st, Tw, wi, pars[blockIdx.y], blockIdx.x,
                                                     blockIdx.y);
 }
+
+#pragma clang fp contract(fast)
 
 // slot -> field map of the rotated flux frame (update.cpp:340-352, 429-441)
 template <int DIR> __device__ __forceinline__ int slot_field(int s) {
@@ -1096,11 +1173,16 @@ static void launch_recon(Ctx* h, int dir, St st, const Real* Tw, Planes10<Real> 
     const unsigned sn = axis_stride(g, an);
     if (dir == 2) {
         Box3 be = sweep_box(g, an, true);
-        const unsigned nI = (unsigned)be.n2;
-        const unsigned span_max = 261u + 5u * ((255u + nI - 1u) / nI);
+        const unsigned NP = (unsigned)be.n2 + 1u;  // the virtual position and the nI interfaces of a line
+        ZPos zp;
+        zp.count = (unsigned)be.n0 * (unsigned)be.n1 * NP;
+        zp.fNP = make_fastdiv(NP);
+        zp.fn1 = make_fastdiv((unsigned)be.n1);
+        const unsigned waves = (zp.count - 1u) / 63u + 1u;  // wavefront w covers [63 w, 63 w + 63]
+        const unsigned span_max = 258u + 4u * ((kZPosBlock + NP - 1u) / NP);
         const size_t lds = 6 * sizeof(Real) * (size_t)span_max;
-        hipLaunchKernelGGL((k_recon_zt<Real, St, RECON, ENERGY>), dim3(blocks_for(be.count(), 256), h->nbox),
-                           dim3(256), lds, h->stream, g, be, st, Tw, wi, h->d_pars, span_max);
+        hipLaunchKernelGGL((k_recon_zc<Real, St, RECON, ENERGY>), dim3((waves + 3u) / 4u, h->nbox), dim3(256), lds,
+                           h->stream, g, zp, st, Tw, wi, h->d_pars, span_max);
         return;
     }
     // X (normal = axis 1) walks i with threads over (j, k); Y (normal = axis 0) walks j over (i, k)
