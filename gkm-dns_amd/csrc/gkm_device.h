@@ -322,7 +322,7 @@ template <class T> __device__ __forceinline__ void full_ntab(const Gas<T>& g, T 
     t.N[0] = T(1);
     t.N[1] = U;
 #pragma unroll
-    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + t.N[i - 2] * T(i - 1) * h;
+    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + (T(i - 1) * h) * t.N[i - 2];
     ntab_sums(g, h, t);
 }
 
@@ -375,7 +375,7 @@ __device__ __forceinline__ void half_ntab(const Gas<T>& g, T U, T lam, T h, int 
     t.N[0] = T(0.5) * erfc_gauss(y, e);
     t.N[1] = U * t.N[0] + T(sgn) * half_rsqrt_pi * e * rs;
 #pragma unroll
-    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + t.N[i - 2] * T(i - 1) * h;
+    for (int i = 2; i < 7; i++) t.N[i] = U * t.N[i - 1] + (T(i - 1) * h) * t.N[i - 2];
     ntab_sums(g, h, t);
 }
 
