@@ -491,6 +491,21 @@ int gkm_set_tuning(gkm_t* hh, int flux_block, int recon_chunk) {
     return GKM_OK;
 }
 
+int gkm_set_tuning_z(gkm_t* hh, int zflux_block, int z_strip) {
+    Ctx* h = ctx(hh);
+    if (!h) return fail(GKM_E_ARG, "null handle");
+    if (zflux_block != 64 && zflux_block != 128 && zflux_block != 256)
+        return fail(GKM_E_ARG, "zflux_block must be 64, 128 or 256 (got %d)", zflux_block);
+    if (z_strip != 4 && z_strip != 8 && z_strip != 16 && z_strip != 32 && z_strip != 64)
+        return fail(GKM_E_ARG, "z_strip must be 4, 8, 16, 32 or 64 (got %d)", z_strip);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->zflux_block = zflux_block;
+    h->z_strip = z_strip;
+    h->tune_gen++;  // groups holding the handle capture their step again
+    drop_graph(h);
+    return GKM_OK;
+}
+
 int gkm_sync(gkm_t* hh) {
     Ctx* h = ctx(hh);
     if (!h) return fail(GKM_E_ARG, "null handle");

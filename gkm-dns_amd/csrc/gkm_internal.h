@@ -30,7 +30,8 @@ struct Ctx {
     int device;
     hipStream_t stream;
     int flux_block = 256;  // X and Y flux kernels (gkm_set_tuning)
-    int zflux_block = 128;  // the fused Z kernel: 128 (mixed) or 64 (FP64), set at creation
+    int zflux_block = 128;  // the fused Z kernel: 128 (mixed) or 64 (FP64) at creation (gkm_set_tuning_z)
+    int z_strip = 16;       // the fused Z kernel's z lines go in strips of this many columns of i
     int recon_chunk = 8;
     unsigned tune_gen = 0;  // counts gkm_set_tuning calls (group captures are keyed by it)
     // per-box step parameters: device array read by the kernels, pinned staging

@@ -792,7 +792,6 @@ struct ZLines {
     unsigned nfull;   // strips of width W
 };
 
-constexpr unsigned kZStrip = 16;
 
 // line l -> interior column (j, i)
 __device__ __forceinline__ void zline_column(const ZLines& zl, unsigned l, unsigned& j, unsigned& i) {
@@ -1136,13 +1135,13 @@ static inline int normal_axis(int dir) { return dir == 0 ? 1 : (dir == 1 ? 0 : 2
 
 static inline unsigned axis_stride(const Geo& g, int a) { return a == 0 ? g.s0 : (a == 1 ? g.s1 : 1u); }
 
-// z lines of the fused Z kernel, in strips of kZStrip columns of i
-static ZLines make_zlines(const Geo& g) {
+// z lines of the fused Z kernel, in strips of `strip` columns of i
+static ZLines make_zlines(const Geo& g, unsigned strip) {
     ZLines zl;
     const unsigned nI = (unsigned)g.nz + 1;
     zl.count = (unsigned)g.ny * (unsigned)g.nx * nI;
     zl.fnI = make_fastdiv(nI);
-    const unsigned W = std::min(kZStrip, (unsigned)g.nx);
+    const unsigned W = std::min(strip, (unsigned)g.nx);
     zl.nfull = (unsigned)g.nx / W;
     const unsigned wl = (unsigned)g.nx - zl.nfull * W;
     zl.fstrip = make_fastdiv(W * (unsigned)g.ny);
@@ -1272,7 +1271,7 @@ static void launch_sweep_z_update(Ctx* h, St st, St nx, const Real* Tw, const Re
     int t1, t2;
     tangential_axes(2, h->par.zbug_compat, t1, t2);
     const unsigned st1 = axis_stride(g, t1), st2 = axis_stride(g, t2);
-    const ZLines zl = make_zlines(g);
+    const ZLines zl = make_zlines(g, (unsigned)h->z_strip);
     const unsigned waves = (zl.count - 1 + 62) / 63;  // wavefront w covers [63 w, 63 w + 63]
     const int bs = h->zflux_block;
     const dim3 grid(blocks_for(waves * 64u, bs), h->nbox);
@@ -1493,7 +1492,7 @@ static void fill_pclass(Ctx* h, const St& cur, const St& nxt, Real* tau0, Real* 
     }
     C.zb = sweep_box(g, 2, true);
     C.nblk[4] = blocks_for(C.zb.count(), kPersistThreads);
-    C.zl = make_zlines(g);
+    C.zl = make_zlines(g, (unsigned)h->z_strip);
     int t1, t2;
     tangential_axes(2, h->par.zbug_compat, t1, t2);
     C.zst1 = axis_stride(g, t1);

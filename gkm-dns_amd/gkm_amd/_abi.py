@@ -33,7 +33,7 @@ EXPORTS = (
     "gkm_compute_dt", "gkm_stats", "gkm_sync", "gkm_set_timing", "gkm_last_timing", "gkm_flux_points",
     "gkm_weno_points", "gkm_last_error", "gkm_version",
     "gkm_create_batch", "gkm_batch_size", "gkm_set_state_box", "gkm_get_state_box", "gkm_step_batch",
-    "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_halo_mode",
+    "gkm_compute_dt_box", "gkm_stats_box", "gkm_set_graph", "gkm_set_tuning", "gkm_set_tuning_z", "gkm_set_halo_mode",
     "gkm_print_stats", "gkm_print_stats_box", "gkm_step_print_stats", "gkm_format_print_stats", "gkm_gks_points",
     "gkm_last_sweep_timing", "gkm_get_grid", "gkm_get_params_box", "gkm_set_params_box",
     "gkm_set_dt_batch", "gkm_group_create", "gkm_group_step", "gkm_group_step_persistent", "gkm_group_sync",
@@ -160,6 +160,7 @@ def load() -> C.CDLL:
     L.gkm_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmStats)]
     L.gkm_set_graph.argtypes = [vp, C.c_int]
     L.gkm_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.gkm_set_tuning_z.argtypes = [vp, C.c_int, C.c_int]
     L.gkm_set_halo_mode.argtypes = [vp, C.c_int]
     L.gkm_print_stats.argtypes = [vp, C.POINTER(GkmPrintStats)]
     L.gkm_print_stats_box.argtypes = [vp, C.c_int, C.POINTER(GkmPrintStats)]

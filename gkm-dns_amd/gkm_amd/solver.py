@@ -292,8 +292,13 @@ class Solver:
         """Capture one step in a hipGraph and replay it (gkm_set_graph)."""
         _abi.check(self._L.gkm_set_graph(self._h, int(on)))
 
-    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
+    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8, zflux_block: int | None = None,
+                   z_strip: int | None = None) -> None:
+        """Launch shapes (gkm_set_tuning; with zflux_block and z_strip also
+        gkm_set_tuning_z).  Results never depend on them."""
         _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
+        if zflux_block is not None:
+            _abi.check(self._L.gkm_set_tuning_z(self._h, int(zflux_block), int(z_strip or 16)))
 
     def set_halo_mode(self, mode: str = "gather") -> None:
         """"gather" (periodic image per ghost cell) or "pack" (y-face pack /
@@ -421,8 +426,11 @@ class Batch:
     def set_graph(self, on: bool = True) -> None:
         _abi.check(self._L.gkm_set_graph(self._h, int(on)))
 
-    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8) -> None:
+    def set_tuning(self, flux_block: int = 256, recon_chunk: int = 8, zflux_block: int | None = None,
+                   z_strip: int | None = None) -> None:
         _abi.check(self._L.gkm_set_tuning(self._h, int(flux_block), int(recon_chunk)))
+        if zflux_block is not None:
+            _abi.check(self._L.gkm_set_tuning_z(self._h, int(zflux_block), int(z_strip or 16)))
 
     def set_halo_mode(self, mode: str = "gather") -> None:
         _abi.check(self._L.gkm_set_halo_mode(self._h, _abi.HALO[mode]))

@@ -208,10 +208,18 @@ int gkm_group_destroy(gkm_group_t* g);
 
 /* Launch shapes: threads per block of the X and Y flux kernels (64, 128 or
  * 256) and interfaces per thread of the strided-sweep WENO kernels (4, 8 or
- * 16).  Chosen per box size by the autotuner in gkm_amd/tune.py.  (The fused
- * Z sweep's block size is fixed per precision.)  Drops the handle's captured
- * step; groups holding the handle capture theirs again on their next step. */
+ * 16).  Chosen per box size by the autotuner in gkm_amd/tune.py.  Drops the
+ * handle's captured step; groups holding the handle capture theirs again on
+ * their next step.  Results never depend on the launch shape.  No reference
+ * counterpart (the reference's launch shapes are compile-time constants,
+ * weno/src/GPUupdate.cu:65-196). */
 int gkm_set_tuning(gkm_t* h, int flux_block, int recon_chunk);
+/* The fused Z sweep's launch shape: threads per block (64, 128 or 256;
+ * created as 128 mixed, 64 FP64) and the width of the strips of columns its
+ * z lines are taken in (4, 8, 16, 32 or 64; created as 16), so that a line's
+ * neighbours are read while they are still in one XCD's L2.  Same rules and
+ * the same autotuner as gkm_set_tuning.  No reference counterpart. */
+int gkm_set_tuning_z(gkm_t* h, int zflux_block, int z_strip);
 
 /* Copy the padded host state in / out (bit-exact round trip for FP64; the
  * mixed form splits each value into an FP32 hi/lo pair and rebuilds it).

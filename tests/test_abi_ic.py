@@ -176,6 +176,8 @@ def test_null_handles_are_rejected():
     assert L.gkm_batch_size(None, C.byref(n)) == A.GKM_E_ARG
     assert L.gkm_set_graph(None, 1) == A.GKM_E_ARG
     assert L.gkm_set_timing(None, 1) == A.GKM_E_ARG
+    assert L.gkm_set_tuning(None, 256, 8) == A.GKM_E_ARG
+    assert L.gkm_set_tuning_z(None, 128, 16) == A.GKM_E_ARG
 
 
 def test_tuning_cache_key_names_the_built_library():

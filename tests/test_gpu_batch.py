@@ -60,12 +60,17 @@ def test_graph_replay_matches_eager(gk, prec):
     g.close()
 
 
-def test_tuning_does_not_change_results(gk):
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_tuning_does_not_change_results(gk, prec):
+    """Every launch shape the autotuner may pick gives the same bits: X/Y flux
+    block and WENO chunk (gkm_set_tuning), the fused Z kernel's block and strip
+    width (gkm_set_tuning_z; 20 columns of i make the last strip partial for
+    widths 8 and 16, and 32 is wider than the box)."""
     ic = gk.dhit(20, Mt=0.5, Re=70, seed=9)
     outs = []
-    for fb, rc in ((256, 8), (64, 4), (128, 16)):
-        s = gk.Solver.from_ic(ic, precision="mixed", recon="js")
-        s.set_tuning(fb, rc)
+    for shape in ((256, 8), (64, 4), (128, 16), (256, 8, 64, 8), (64, 16, 256, 32), (128, 4, 128, 4)):
+        s = gk.Solver.from_ic(ic, precision=prec, recon="js")
+        s.set_tuning(*shape)
         s.evolve_eulerian(3, dt=0.01)
         outs.append(s.get_state())
         s.close()
