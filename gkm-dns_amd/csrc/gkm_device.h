@@ -7,7 +7,7 @@
 // and time-integral coefficients) but is reorganised:
 //  * each Maxwellian's moments are taken in its tangentially co-moving frame,
 //    where they collapse to a few products of the normal-axis table (see the
-//    block comment above acc_u); the reference's MCal instead expands every
+//    block comment above Maxw); the reference's MCal instead expands every
 //    <u^k v^l w^m psi (a.psi)> in the lab frame (~85 flops, 35 calls per
 //    interface);
 //  * tau and the gamma / p time-integral coefficients are formed first, then
@@ -16,7 +16,10 @@
 //    slopes die as soon as its block ends (the reference keeps ~400 doubles
 //    live, GPUupdate.cu:499-520);
 //  * only the normal axis carries half-space moments (the reference fills them
-//    for all three axes, but MCal reads them only from row 0).
+//    for all three axes, but MCal reads them only from row 0);
+//  * a side's time-slope, time-source and flux moments are summed over their
+//    four slope vectors component by component before the table is applied
+//    (g_coef / g_mom / g_pair), and adjacent moment orders share a component.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -26,7 +29,7 @@
 namespace gkm {
 
 template <class T> struct M;
-// FP64: the ROCm device library for erfc/exp/pow; reciprocals from v_rcp_f64
+// FP64: the ROCm device library for exp and pow; reciprocals from v_rcp_f64
 // refined by two Newton steps (within an ulp of 1/x for the normal-range
 // values that occur here; the IEEE division sequence is about twice as long).
 template <> struct M<double> {
@@ -379,40 +382,16 @@ __device__ __forceinline__ void half_ntab(const Gas<T>& g, T U, T lam, T h, int 
     ntab_sums(g, h, t);
 }
 
-// acc += s <u^k psi' (a . psi')>
-template <int k, class T>
-__device__ __forceinline__ void acc_u(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
-    acc[0] += s * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Q[k]);
-    acc[1] += s * (a[0] * t.N[k + 1] + a[1] * t.N[k + 2] + a[4] * t.Q[k + 1]);
-    const T sh = s * h * t.N[k];
-    acc[2] += sh * a[2];
-    acc[3] += sh * a[3];
-    acc[4] += s * (a[0] * t.Q[k] + a[1] * t.Q[k + 1] + a[4] * t.R[k]);
-}
-
-// acc += s <u^k v' psi' (a . psi')>
-template <int k, class T>
-__device__ __forceinline__ void acc_v(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
-    const T sh = s * h;
-    acc[0] += sh * a[2] * t.N[k];
-    acc[1] += sh * a[2] * t.N[k + 1];
-    acc[2] += sh * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Qp[k]);
-    acc[4] += sh * a[2] * t.Qp[k];
-}
-
-// acc += s <u^k w' psi' (a . psi')>
-template <int k, class T>
-__device__ __forceinline__ void acc_w(T acc[5], T s, const NTab<T>& t, T h, const T a[5]) {
-    const T sh = s * h;
-    acc[0] += sh * a[3] * t.N[k];
-    acc[1] += sh * a[3] * t.N[k + 1];
-    acc[3] += sh * (a[0] * t.N[k] + a[1] * t.N[k + 1] + a[4] * t.Qp[k]);
-    acc[4] += sh * a[3] * t.Qp[k];
-}
-
+// The moments of a slope vector a (psi' basis) against one normal table:
+//   u_k(a) = <u^k psi' (a . psi')>
+//          = (a0 N_k + a1 N_k+1 + a4 Q_k,  a0 N_k+1 + a1 N_k+2 + a4 Q_k+1,  h N_k a2,  h N_k a3,
+//             a0 Q_k + a1 Q_k+1 + a4 R_k)
+//   v_k(a) = <u^k v' psi' (a . psi')> = h (a2 N_k,  a2 N_k+1,  a0 N_k + a1 N_k+1 + a4 Q'_k,  0,  a2 Q'_k)
+//   w_k(a) = <u^k w' psi' (a . psi')> = h (a3 N_k,  a3 N_k+1,  0,  a0 N_k + a1 N_k+1 + a4 Q'_k,  a3 Q'_k)
+// (the odd tangential Gaussian moments vanish; <v'^2> = h).
 // The flux's time-slope, time-source and flux moments are all of the form
 //   G_k = <u^k psi' (u ax + v' ay + w' az) . psi'> + <u^k psi' (sv . psi')>
-//       = u_{k+1}(ax) + v_k(ay) + w_k(az) + u_k(sv)         (acc_u/v/w above)
+//       = u_{k+1}(ax) + v_k(ay) + w_k(az) + u_k(sv)         (u_k, v_k, w_k above)
 // with sv = V ay + W az.  Summed component by component, the four terms share
 // their table entries, so G_k is five short dot products of the table with
 // coefficients formed once per (ax, ay, az, sv):
