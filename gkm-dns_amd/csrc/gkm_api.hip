@@ -309,7 +309,7 @@ int gkm_set_params_box(gkm_t* hh, int box, const gkm_params* p) {
     h->bpar[box] = *p;
     if (box == 0) h->par = *p;
     h->pars_valid = false;
-    h->derived_valid = false;  // Tw and tau depend on the gas constants
+    gkm::invalidate_derived(h);  // Tw and tau depend on the gas constants
     return GKM_OK;
 }
 
@@ -340,7 +340,7 @@ int gkm_set_state_box(gkm_t* hh, int box, const double* const W[5]) {
         if (!W[q]) return fail(GKM_E_ARG, "null state plane %d", q);
     const size_t bytes = sizeof(double) * (size_t)h->ncell;
     const size_t off = (size_t)box * h->ncell;
-    h->derived_valid = false;
+    gkm::invalidate_derived(h);
     HIP_TRY(hipMemsetAsync(h->d_bad + box, 0, sizeof(unsigned), h->stream));
     if (h->par.precision == GKM_PREC_FP64) {
         for (int q = 0; q < 5; q++)
@@ -433,7 +433,7 @@ int gkm_step_batch(gkm_t* hh, const double* dt, int nsteps) {
                     if (gr) (void)hipGraphDestroy(gr);
                     h->graph_exec[f] = nullptr;
                     gkm::swap_state(h);
-                    h->derived_valid = false;
+                    gkm::invalidate_derived(h);
                     return fail(GKM_E_HIP, "captured step failed: %s", hipGetErrorString(e));
                 }
                 h->graph[f] = gr;
@@ -660,7 +660,7 @@ int gkm_group_step(gkm_group_t* g, int nsteps) {
                 if (c.graph) (void)hipGraphDestroy(c.graph);
                 for (Ctx* h : g->hs) {
                     gkm::swap_state(h);
-                    h->derived_valid = false;
+                    gkm::invalidate_derived(h);
                 }
                 return fail(GKM_E_HIP, "group step capture failed: %s", hipGetErrorString(e));
             }
@@ -874,7 +874,7 @@ int gkm_set_halo_mode(gkm_t* hh, int mode) {
         HIP_TRY(hipMalloc(&h->halo_recv, bytes));
     }
     h->halo_mode = mode;
-    h->derived_valid = false;
+    gkm::invalidate_derived(h);
     drop_graph(h);
     return GKM_OK;
 }

@@ -69,8 +69,13 @@ struct Ctx {
     // the current buffer's ghost cells, and for the WENO-BGK scheme its Tw and
     // tau planes, are those of its interior (written by the last step's Z sweep
     // or by a ghost fill); anything that changes the state or the gas
-    // constants clears it
+    // constants clears it (invalidate_derived)
     bool derived_valid = false;
+    // GKM_HALO_PACK: the current buffer's Tw and tau hold its interior cells'
+    // values (the last WENO-BGK step's Z sweep wrote them), so a step only
+    // recomputes the ghost shell's after the exchange; cleared with
+    // derived_valid
+    bool prims_valid = false;
     // non-physical flags, one per box, raised by the fused Z kernel on a new
     // cell with rho or T not positive and finite; cleared by gkm_set_state_box.
     // Every gkm_step_batch call ends with a copy of the flags to the pinned
@@ -121,6 +126,11 @@ void ensure_ghosts(Ctx* h);             // ghosts valid for a read of the state
 // buffer and swaps the current-state pointers (swap_state)
 void launch_step(Ctx* h, hipEvent_t* ev);  // ev: kEvPerStep events or nullptr
 void swap_state(Ctx* h);
+// the state, the gas constants or the halo mode changed: ghosts and per-cell primitives are stale
+inline void invalidate_derived(Ctx* h) {
+    h->derived_valid = false;
+    h->prims_valid = false;
+}
 // persistent group step: the size of one class entry, an entry made from a
 // handle's current buffers, and the cooperative launch over ncls entries
 // (device table) for nsteps steps; grid_out receives the block count

@@ -145,6 +145,43 @@ __global__ void __launch_bounds__(256) k_prim(Geo g, St st, Real* Tw, Real* tau,
     put(tau, t, tc);
 }
 
+// The per-cell primitives of the ghost shell only (GKM_HALO_PACK, after the
+// exchange; the Z sweep wrote the interior's): t runs over the 6 P1 P2 cells
+// of the y ghost layers, then the 6 ny P2 of the x ghost layers inside them,
+// then the 6 ny nx of the z ghost layers inside both (each ghost cell once).
+template <class Real, class St>
+__global__ void __launch_bounds__(256) k_prim_shell(Geo g, St st, Real* Tw, Real* tau,
+                                                    const StepPar* __restrict__ pars) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned P0 = (unsigned)g.P0, P1 = (unsigned)g.P1, P2 = (unsigned)g.P2;
+    const unsigned ny = (unsigned)g.ny, nx = (unsigned)g.nx;
+    const unsigned n1 = 6u * P1 * P2, n2 = 6u * ny * P2, n3 = 6u * ny * nx;
+    if (t >= n1 + n2 + n3) return;
+    unsigned j, i, k;
+    if (t < n1) {
+        const unsigned layer = t / (P1 * P2), r = t - layer * (P1 * P2);
+        j = layer < 3u ? layer : P0 - 6u + layer;
+        i = r / P2;
+        k = r - i * P2;
+    } else if (t < n1 + n2) {
+        const unsigned u = t - n1, row = u / (6u * P2), v = u - row * (6u * P2), il = v / P2;
+        j = 3u + row;
+        i = il < 3u ? il : P1 - 6u + il;
+        k = v - il * P2;
+    } else {
+        const unsigned u = t - n1 - n2, row = u / (6u * nx), w = u - row * (6u * nx), kl = w % 6u;
+        j = 3u + row;
+        i = 3u + w / 6u;
+        k = kl < 3u ? kl : P2 - 6u + kl;
+    }
+    const unsigned c = blockIdx.y * g.ncell + j * g.s0 + i * g.s1 + k;
+    const StepPar& par = pars[blockIdx.y];
+    Real Tc, tc;
+    cell_prims<Real>(par, st.get(0, c), st.get(1, c), st.get(2, c), st.get(3, c), st.get(4, c), Tc, tc);
+    put(Tw, c, Tc);
+    put(tau, c, tc);
+}
+
 // ---------------------------------------------------------------------------
 // WENO reconstruction of one sweep on the extended interface set (normal
 // index [2, P-4], tangential [2, P-3]): each interface once.  The reference
@@ -861,8 +898,9 @@ __device__ __forceinline__ void flux_update_block(const Geo& g, const ZLines& zl
     CellVal<St> v;
     update_finish(c, nx, Fz, Fm, upd_lds, par, v);
     bool ok;
-    if constexpr (GHOST && SCHEME != GKM_SCHEME_GKS) {
-        ok = ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
+    if constexpr (SCHEME != GKM_SCHEME_GKS) {
+        // the cell's Tw and tau for the next step, and (GHOST) its ghost images
+        ok = ghost_images<Real, St, SCHEME>(g, GHOST ? img_lds[threadIdx.x] : 0u, c, nx, v, Tw_nx, tau_nx, par);
     } else {
         if constexpr (GHOST) ghost_images<Real, St, SCHEME>(g, img_lds[threadIdx.x], c, nx, v, Tw_nx, tau_nx, par);
         ok = physical<St>(par, v);
@@ -1309,7 +1347,10 @@ void swap_state(Ctx* h) {
     std::swap(h->taud, h->taud_nx);
     std::swap(h->tauf, h->tauf_nx);
     h->flip ^= 1;
-    h->derived_valid = h->halo_mode != GKM_HALO_PACK;
+    const bool pack = h->halo_mode == GKM_HALO_PACK;
+    h->derived_valid = !pack;
+    // in the slab exchange the Z sweep wrote the interior's Tw and tau
+    h->prims_valid = pack && h->par.flux_scheme != GKM_SCHEME_GKS;
 }
 
 void launch_ghost(Ctx* h, bool prim) {
@@ -1357,21 +1398,34 @@ void launch_ghost(Ctx* h, bool prim) {
     }
 }
 
-// per-cell primitives when the ghost fill did not compute them (GKM_HALO_PACK)
-static void launch_prim(Ctx* h) {
+// per-cell primitives when the ghost fill did not compute them (GKM_HALO_PACK):
+// every padded cell, or (shell) the ghost shell only
+static void launch_prim(Ctx* h, bool shell = false) {
     const Geo& g = h->geo;
-    const dim3 grid(blocks_for(g.ncell, 256), h->nbox);
+    const unsigned nshell =
+        6u * (unsigned)(g.P1 * g.P2) + 6u * (unsigned)(g.ny * g.P2) + 6u * (unsigned)(g.ny * g.nx);
+    const dim3 grid(blocks_for(shell ? nshell : g.ncell, 256), h->nbox);
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st;
         for (int q = 0; q < 5; q++) st.w[q] = h->Wd[q];
-        hipLaunchKernelGGL((k_prim<double, StateD>), grid, dim3(256), 0, h->stream, g, st, h->Td, h->taud, h->d_pars);
+        if (shell)
+            hipLaunchKernelGGL((k_prim_shell<double, StateD>), grid, dim3(256), 0, h->stream, g, st, h->Td, h->taud,
+                               h->d_pars);
+        else
+            hipLaunchKernelGGL((k_prim<double, StateD>), grid, dim3(256), 0, h->stream, g, st, h->Td, h->taud,
+                               h->d_pars);
     } else {
         StateM st;
         for (int q = 0; q < 5; q++) {
             st.hi[q] = h->Whi[q];
             st.lo[q] = h->Wlo[q];
         }
-        hipLaunchKernelGGL((k_prim<float, StateM>), grid, dim3(256), 0, h->stream, g, st, h->Tf, h->tauf, h->d_pars);
+        if (shell)
+            hipLaunchKernelGGL((k_prim_shell<float, StateM>), grid, dim3(256), 0, h->stream, g, st, h->Tf, h->tauf,
+                               h->d_pars);
+        else
+            hipLaunchKernelGGL((k_prim<float, StateM>), grid, dim3(256), 0, h->stream, g, st, h->Tf, h->tauf,
+                               h->d_pars);
     }
 }
 
@@ -1394,9 +1448,16 @@ StepPar step_par(const gkm_grid& grid, const gkm_params& pp, double dt) {
 }
 
 void prepare_step(Ctx* h) {
-    if (h->halo_mode != GKM_HALO_PACK && !h->derived_valid) {
-        launch_ghost(h, h->par.flux_scheme != GKM_SCHEME_GKS);
-        h->derived_valid = true;
+    const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
+    if (h->halo_mode != GKM_HALO_PACK) {
+        if (!h->derived_valid) {
+            launch_ghost(h, prim);
+            h->derived_valid = true;
+        }
+    } else if (prim && !h->prims_valid) {
+        // the interior's Tw and tau (the step recomputes the ghost shell's after its exchange)
+        launch_prim(h);
+        h->prims_valid = true;
     }
 }
 
@@ -1411,16 +1472,17 @@ void launch_step(Ctx* h, hipEvent_t* evs) {
     // the periodic fill the Z sweep also writes that buffer's ghosts and
     // primitives, and the ghost fill runs only when they are not valid (after
     // set_state and the like); with the slab exchange (GKM_HALO_PACK) every
-    // step starts with the exchange.
+    // step starts with the exchange, and the Z sweep writes the interior's
+    // primitives, so after the exchange only the ghost shell's are formed.
+    // (prepare_step is also called before a captured step, outside the graph,
+    // so that the graph holds the same work on every replay.)
     const bool prim = h->par.flux_scheme != GKM_SCHEME_GKS;
     const bool pack = h->halo_mode == GKM_HALO_PACK;
     ev(0);
-    if (pack)
-        launch_ghost(h, prim);
-    else
-        prepare_step(h);
+    prepare_step(h);
+    if (pack) launch_ghost(h, prim);
     ev(12);
-    if (prim && pack) launch_prim(h);
+    if (prim && pack) launch_prim(h, true);
     ev(1);
     if (h->par.precision == GKM_PREC_FP64) {
         StateD st, nx;
