@@ -291,8 +291,10 @@ int gkm_sync(gkm_t* h);
  * handle's stream around each launch (first 64 steps of the call):
  * ms[0] BGK flux kernels, ms[1] WENO reconstruction kernels, ms[2] ghost
  * fill (in GKM_HALO_GATHER mode fused with the primitives), ms[3]
- * conservative updates, ms[4] the separate primitives kernel of
- * GKM_HALO_PACK mode (0 otherwise); -1 where nothing was recorded.
+ * conservative updates, ms[4] the ghost shell's primitives in
+ * GKM_HALO_PACK mode (the fused Z sweep forms the interior's; a full-box pass
+ * after a state change counts in ms[2]; 0 in GKM_HALO_GATHER mode); -1 where
+ * nothing was recorded.
  * *nsteps receives the number of timed steps.  Timing is off by default.
  * mode: GKM_TIMING_ALL (any other nonzero value too) records every event;
  * GKM_TIMING_FLUX records only those around the X and Y flux kernels, so the
