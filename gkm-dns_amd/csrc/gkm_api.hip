@@ -102,6 +102,7 @@ void free_ctx(Ctx* h) {
     if (h->spl_part) (void)hipFree(h->spl_part);
     if (h->h_pars) (void)hipHostFree(h->h_pars);
     if (h->ps_host) (void)hipHostFree(h->ps_host);
+    if (h->ps_row) (void)hipHostFree(h->ps_row);
     if (h->h_bad) (void)hipHostFree(h->h_bad);
     if (h->ev_bad) (void)hipEventDestroy(h->ev_bad);
     if (h->arena) (void)hipFree(h->arena);
@@ -957,9 +958,10 @@ int gkm_stats(gkm_t* h, gkm_stats_t* out) { return gkm_stats_box(h, 0, out); }
 
 // the nine gradient planes of one box, the Thomas pivots of the three line
 // lengths and the backward-sum weights of the y and x lines, allocated on
-// first use
+// first use (the pinned row last: a call that failed part way allocates the
+// rest on the next one)
 static int ensure_spline_buffers(Ctx* h) {
-    if (h->spl_grad) return GKM_OK;
+    if (h->ps_row) return GKM_OK;
     const gkm::Geo& g = h->geo;
     // Thomas pivots of the natural-spline slope system for each line length
     std::vector<double> inv;
@@ -990,17 +992,20 @@ static int ensure_spline_buffers(Ctx* h) {
         }
         first += (size_t)m;
     }
-    hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
-    if (e != hipSuccess) {
-        h->spl_grad = nullptr;
-        return fail(GKM_E_NOMEM, "gradient planes (%zu bytes): %s", sizeof(double) * 9 * (size_t)h->ncell,
-                    hipGetErrorString(e));
+    if (!h->spl_grad) {
+        hipError_t e = hipMalloc((void**)&h->spl_grad, sizeof(double) * 9 * (size_t)h->ncell);
+        if (e != hipSuccess) {
+            h->spl_grad = nullptr;
+            return fail(GKM_E_NOMEM, "gradient planes (%zu bytes): %s", sizeof(double) * 9 * (size_t)h->ncell,
+                        hipGetErrorString(e));
+        }
     }
-    HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
+    if (!h->spl_inv) HIP_TRY(hipMalloc((void**)&h->spl_inv, sizeof(double) * inv.size()));
     HIP_TRY(hipMemcpy(h->spl_inv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc((void**)&h->spl_wt, sizeof(double) * wt.size()));
+    if (!h->spl_wt) HIP_TRY(hipMalloc((void**)&h->spl_wt, sizeof(double) * wt.size()));
     HIP_TRY(hipMemcpy(h->spl_wt, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc((void**)&h->spl_part, sizeof(double) * gkm::print_stats_part_doubles(g)));
+    if (!h->spl_part) HIP_TRY(hipMalloc((void**)&h->spl_part, sizeof(double) * gkm::print_stats_part_doubles(g)));
+    HIP_TRY(hipHostMalloc((void**)&h->ps_row, sizeof(double) * (gkm::kPStat + gkm::kStatB), hipHostMallocDefault));
     return GKM_OK;
 }
 
@@ -1045,10 +1050,12 @@ int gkm_print_stats_box(gkm_t* hh, int box, gkm_print_stats_t* out) {
     rc = ensure_spline_buffers(h);
     if (rc) return rc;
     gkm::ensure_ghosts(h);
-    double a[gkm::kPStat + gkm::kStatB];
+    // the sums and the non-physical flags into pinned memory, one wait for both
+    double* a = h->ps_row;
     HIP_TRY(gkm::launch_print_stats_async(h, box, h->stream, a));
-    HIP_TRY(hipStreamSynchronize(h->stream));  // the sums land in a
-    rc = check_physical(h);
+    HIP_TRY(hipMemcpyAsync(h->h_bad, h->d_bad, sizeof(unsigned) * h->nbox, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    rc = bad_report(h);
     const int rs = fill_print_stats(h, a, a + gkm::kPStat, out);
     return rc ? rc : rs;
 }

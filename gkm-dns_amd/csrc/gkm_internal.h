@@ -103,6 +103,7 @@ struct Ctx {
     hipEvent_t ev_state = nullptr, ev_ps[2] = {nullptr, nullptr};
     double* ps_host = nullptr;
     int ps_rows = 0;
+    double* ps_row = nullptr;  // pinned: the row of one gkm_print_stats call
     // halo mode and the y-face exchange buffers of GKM_HALO_PACK
     int halo_mode = GKM_HALO_GATHER;
     void* halo_send = nullptr;

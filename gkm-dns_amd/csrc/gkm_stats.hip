@@ -448,6 +448,7 @@ __global__ void __launch_bounds__(kSpThreads) k_spl_chunked(Geo g, St st, const 
 // also stores and reloads y), and every load and store runs along the lines,
 // which are contiguous.
 constexpr int kSplL = 8, kSplThreads = 192;
+constexpr int kSplLoadLines = 4;  // lines whose state words are loaded together
 constexpr int kSplMaxM = 341;  // 3 kSplL m doubles within 64 KB
 
 // line t of axis A (lines over the two other axes' interiors, the faster
@@ -468,14 +469,13 @@ template <int A> struct SplAxis {
 // t takes node t + 192 k of all 8 lines.
 //
 // spl_load: the velocities of the group's nodes into the tile.  A batch's
-// loads (one node of the 8 lines) are all issued before its divisions and LDS
-// writes, with no branch around an element: a missing line (the last group's)
-// repeats the group's last line, whose slot is never read.
+// loads (one node of kSplLoadLines lines) are all issued before its divisions
+// and LDS writes, with no branch around an element: a missing line (the last
+// group's) repeats the group's last line, whose slot is never read.
 template <class St, int A>
 __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAxis<A>& ax, unsigned t0,
                                          unsigned nl, double* tile) {
     const int m = ax.m, t = threadIdx.x;
-    constexpr int K = 8;
     // the batch's raw words (the mixed state's hi and lo floats) are loaded
     // into registers first and only then combined: combining each value as
     // it is loaded had the compiler wait for every pair of loads in turn
@@ -498,95 +498,124 @@ __device__ __forceinline__ void spl_load(const Geo& g, const St& st, const SplAx
     };
     {
         static_assert(A == 2, "x and y lines go through k_spl_stream");
-        static_assert(K == kSplL, "one batch = one node of the 8 lines");
+        static_assert(kSplL % kSplLoadLines == 0, "whole batches");
         unsigned b[kSplL];
 #pragma unroll
         for (int l = 0; l < kSplL; l++) b[l] = ax.base(g, min(t0 + l, nl - 1));
         for (int n = t; n < m; n += kSplThreads) {
-            Raw h[kSplL][4], lo[kSplL][4];
 #pragma unroll
-            for (int l = 0; l < kSplL; l++)
+            for (int l0 = 0; l0 < kSplL; l0 += kSplLoadLines) {
+                Raw h[kSplLoadLines][4], lo[kSplLoadLines][4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) raw(q, b[l] + (unsigned)n, h[l][q], lo[l][q]);
+                for (int l = 0; l < kSplLoadLines; l++)
 #pragma unroll
-            for (int l = 0; l < kSplL; l++) {
-                double v[4];
+                    for (int q = 0; q < 4; q++) raw(q, b[l0 + l] + (unsigned)n, h[l][q], lo[l][q]);
 #pragma unroll
-                for (int q = 0; q < 4; q++) v[q] = (double)h[l][q] + (double)lo[l][q];
-                put_f(l, n, v);
+                for (int l = 0; l < kSplLoadLines; l++) {
+                    double v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = (double)h[l][q] + (double)lo[l][q];
+                    put_f(l0 + l, n, v);
+                }
             }
         }
-    }
-}
-
-// spl_store: body(l, n, c) for the group's existing lines, nodes [n0, n1)
-template <int A, class F>
-__device__ __forceinline__ void spl_store(const Geo& g, const SplAxis<A>& ax, unsigned t0, unsigned nl, int n0,
-                                          int n1, F&& body) {
-    static_assert(A == 2, "x and y lines go through k_spl_stream");
-    const int t = threadIdx.x;
-    for (int l = 0; l < kSplL; l++) {
-        if (t0 + l >= nl) break;
-        const unsigned b = ax.base(g, t0 + l);
-        for (int n = n0 + t; n < n1; n += kSplThreads) body(l, n, b + (unsigned)n);
     }
 }
 
 // The group's velocities into tile[(c m + n) kSplL + l] (c = component),
-// then lane (c, l) of the first 24 solves its line in place: f -> y -> D.
-template <class St, int A>
+// then the first 48 lanes solve the lines in place, two lanes a line and
+// component, while the wavefronts past the first run side().
+//
+// The solve is the twisted factorization of the line's system: the lane of
+// the upper half eliminates rows 0 .. k-1 downward (the Thomas forward sweep,
+// y_i = (r_i - y_{i-1}) inv_i), the lane of the lower half rows m-1 .. k+1
+// upward (z_i = (r_i - z_{i+1}) inv'_i), k = m/2; row k then gives
+//   D_k = (r_k - y_{k-1} - z_{k+1}) / (4 - inv_{k-1} - inv'_{k+1}),
+// and each lane substitutes back outward from it (D_i = y_i - inv_i D_{i+1},
+// D_i = z_i - inv'_i D_{i-1}).  The system reads the same from either end, so
+// inv'_i = inv_{m-1-i}: both lanes run the same recursions in their own
+// local node order (j = i upward, j = m-1-i downward, where r changes sign),
+// and the serial chain is half the whole-line walk's.  Neither lane writes a
+// node the other reads: the upper one reads f of nodes 0 .. k and writes
+// 0 .. k, the lower one reads k .. m-1 and writes k+1 .. m-1.  The values
+// differ from the whole-line walk's by rounding only.
+template <class St, int A, class Side>
 __device__ __forceinline__ void spl_group_solve(const Geo& g, const St& st, const SplAxis<A>& ax, unsigned t0,
                                                 unsigned nl, const double* __restrict__ inv, double r3h,
-                                                double* tile) {
+                                                double* tile, Side&& side) {
+    static_assert(6 * kSplL <= 64, "the solving lanes fit one wavefront");
     const int m = ax.m;
     spl_load(g, st, ax, t0, nl, tile);
     __syncthreads();
-    if (threadIdx.x < 3 * kSplL) {
-        const int c = threadIdx.x / kSplL, l = threadIdx.x % kSplL;
-        double* T = tile + (size_t)c * m * kSplL + l;  // node n at T[n * kSplL]
+    // the wavefront's number as a scalar: the branch is taken per wavefront
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x / 64u) != 0u) {
+        side();  // the other wavefronts' work, beside the solve (which reads and writes only the tile)
+    } else if (threadIdx.x < 6 * kSplL) {
+        const int lane = threadIdx.x, hf = lane / (3 * kSplL), cl = lane % (3 * kSplL);
+        const int c = cl / kSplL, l = cl % kSplL;
+        const int k = m / 2;          // the twist row
+        const int nc = m - 1 - k;     // rows both halves eliminate (the upper one one more when m is even)
+        const int st = hf ? -kSplL : kSplL;
+        double* T = tile + (size_t)c * m * kSplL + l + (hf ? (size_t)(m - 1) * kSplL : 0);  // local j at T[j st]
+        const double r3 = hf ? -r3h : r3h;
         constexpr int U = 8;  // LDS reads of U nodes ahead of the dependent steps
-        // forward: node 0 (one-sided), 1 .. m-2 (central), m-1 (one-sided)
-        double fm = T[0], f0 = T[kSplL];
-        double y = (r3h * (f0 - fm)) * inv[0];  // y_{-1} = 0
+        // elimination: local row 0 (one-sided), then the central rows
+        double fm = T[0], f0 = T[st];
+        double y = (r3 * (f0 - fm)) * inv[0];
         T[0] = y;
-        int i = 1;
-        for (; i + U <= m - 1; i += U) {  // f of nodes i+1 .. i+U, all <= m-1
+        int j = 1;
+        for (; j + U <= nc; j += U) {  // f of local nodes j+1 .. j+U, all <= nc
             double fn[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) fn[k] = T[(i + 1 + k) * kSplL];
+            for (int u = 0; u < U; u++) fn[u] = T[(j + 1 + u) * st];
 #pragma unroll
-            for (int k = 0; k < U; k++) {
-                y = (r3h * (fn[k] - fm) - y) * inv[i + k];
-                T[(i + k) * kSplL] = y;
+            for (int u = 0; u < U; u++) {
+                y = (r3 * (fn[u] - fm) - y) * inv[j + u];
+                T[(j + u) * st] = y;
                 fm = f0;
-                f0 = fn[k];
+                f0 = fn[u];
             }
         }
-        for (; i <= m - 2; i++) {
-            const double fp = T[(i + 1) * kSplL];
-            y = (r3h * (fp - fm) - y) * inv[i];
-            T[i * kSplL] = y;
+        for (; j < nc; j++) {
+            const double fp = T[(j + 1) * st];
+            y = (r3 * (fp - fm) - y) * inv[j];
+            T[j * st] = y;
             fm = f0;
             f0 = fp;
         }
-        y = (r3h * (f0 - fm) - y) * inv[m - 1];
-        T[(m - 1) * kSplL] = y;
-        // backward from D_{m-1} = y_{m-1} down to node 3
-        double d = y;
-        int n = m - 2;
+        if (k > nc && hf == 0) {  // the upper half's row k-1 (m even)
+            const double fp = T[(nc + 1) * st];
+            y = (r3 * (fp - fm) - y) * inv[nc];
+            T[nc * st] = y;
+            fm = f0;
+        }
+        // each lane now holds f and its elimination's value at its last row,
+        // k-1 (upper) or k+1 (lower): row k from both, the same bits in both lanes
+        const int partner = hf ? lane - 3 * kSplL : lane + 3 * kSplL;
+        const double fo = __shfl(fm, partner, 64), yo = __shfl(y, partner, 64);
+        const double fa = hf ? fo : fm, ya = hf ? yo : y;  // row k-1
+        const double fb = hf ? fm : fo, zb = hf ? y : yo;  // row k+1
+        double d = (r3h * (fb - fa) - ya - zb) * (1.0 / (4.0 - inv[k - 1] - inv[m - 2 - k]));
+        if (hf == 0) T[k * st] = d;
+        // back-substitution outward from row k down to local node 3
+        if (k > nc && hf == 0) {
+            d = T[nc * st] - inv[nc] * d;
+            T[nc * st] = d;
+        }
+        int n = nc - 1;
         for (; n - (U - 1) >= 3; n -= U) {
             double yv[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) yv[k] = T[(n - k) * kSplL];
+            for (int u = 0; u < U; u++) yv[u] = T[(n - u) * st];
 #pragma unroll
-            for (int k = 0; k < U; k++) {
-                d = yv[k] - inv[n - k] * d;
-                T[(n - k) * kSplL] = d;
+            for (int u = 0; u < U; u++) {
+                d = yv[u] - inv[n - u] * d;
+                T[(n - u) * st] = d;
             }
         }
         for (; n >= 3; n--) {
-            d = T[n * kSplL] - inv[n] * d;
-            T[n * kSplL] = d;
+            d = T[n * st] - inv[n] * d;
+            T[n * st] = d;
         }
     }
     __syncthreads();
@@ -746,11 +775,6 @@ __global__ void __launch_bounds__(kSplStreamThreads) k_spl_stream(Geo g, St st, 
     }
 }
 
-// The fused z pass: the z lines' slopes as k_spl_lines makes them, then, at
-// every interior node of the group, the sums that need the state (rho|u|^2,
-// p, T, 1/rho and their shifted sums) and the z slopes' terms; mu(T), dudz
-// and dvdz go to their planes for the x and y passes.  One row of kPSum
-// partial sums per block.
 // The five state words of a cell, loaded before they are combined (the
 // mixed state's hi and lo floats, or the FP64 values).
 template <class St> struct Raw5;
@@ -791,13 +815,22 @@ __device__ __forceinline__ void cell_fast(const R& r, const StatPar& p, double& 
                          : p.mu0 * x * sqrt(x) * (p.T0 + 110.4) / (T + 110.4);
 }
 
+// The fused z pass.  While the first wavefront solves a group's z lines, the
+// other two take the sums that need only the state (rho|u|^2, p, T, 1/rho
+// and their shifted sums) at the group's interior nodes and write mu(T) to
+// its plane; then every thread adds the z slopes' terms, with mu read back
+// from the plane, and writes dudz, dvdz to their planes for the x and y
+// passes.  One row of kPSum partial sums per block.
+constexpr int kSplBatch = 4;
 template <class St>
-__global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_eu(1, 3)))
+__global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
     k_spl_lines_sums(Geo g, St st, const double* __restrict__ inv, double r3h, StatPar p, int gpb,
                      double* __restrict__ grad, double* __restrict__ part) {
+    static_assert(kSplThreads > 64 && kSplThreads % 64 == 0, "a solving wavefront and side wavefronts");
     extern __shared__ double spl_tile[];
     const SplAxis<2> ax(g);
     const unsigned nl = ax.n_in * ax.n_out;
+    const int m = ax.m, ni = m - 6;
     double acc[kPSum];
 #pragma unroll
     for (int k = 0; k < kPSum; k++) acc[k] = 0.0;
@@ -807,53 +840,65 @@ __global__ void __launch_bounds__(kSplThreads) __attribute__((amdgpu_waves_per_e
         cell_pt(st, 3 * g.s0 + 3 * g.s1 + 3, p, rho, q2, K[0], K[1]);
         K[2] = 1.0 / rho;
     }
+    double* const mu_pl = grad + (size_t)kPlMu * g.ncell;
     SlopeSums sums;
-    // node n of line l at cell c: the state sums and the z slopes' terms;
-    // mu(T), dudz and dvdz to their planes
-    auto body_node = [&](const Raw5<St>& r, unsigned c, int l, int n, int m) {
-        double rho, q2, P, T, iv, mu;
-        cell_fast(r, p, rho, q2, P, T, iv, mu);
-        const double dz[3] = {spl_tile[(0 * m + n) * kSplL + l], spl_tile[(1 * m + n) * kSplL + l],
-                              spl_tile[(2 * m + n) * kSplL + l]};
-        put(grad + (size_t)kPlMu * g.ncell, c, mu);
-        put(grad + (size_t)kPlDudz * g.ncell, c, dz[0]);
-        put(grad + (size_t)kPlDvdz * g.ncell, c, dz[1]);
-        sums.add(dz[0], mu, false, 0.0);  // dudz: its cross term dudz dwdx is added by the x pass
-        sums.add(dz[1], mu, false, 0.0);  // dvdz: dwdy dvdz by the y pass
-        sums.add(dz[2], mu, true, 0.0);
-        acc[0] += rho * q2;
-        acc[12] += P; acc[13] += T; acc[14] += iv;
-        const double dp = P - K[0], dT = T - K[1], dv = iv - K[2];
-        acc[15] += dp; acc[16] += dT; acc[17] += dv;
-        acc[18] += dp * dp; acc[19] += dT * dT; acc[20] += dv * dv;
-    };
     // gpb consecutive groups per block, the blocks of an XCD on neighbouring groups
     for (int grp = 0; grp < gpb; grp++) {
         const unsigned t0 = (xcd_block() * (unsigned)gpb + (unsigned)grp) * kSplL;
         if (t0 >= nl) break;  // block-uniform
         if (grp > 0) __syncthreads();  // the previous group's slopes have been read
-        spl_group_solve(g, st, ax, t0, nl, inv, r3h, spl_tile);
-        const int m = ax.m;
         // the group's interior nodes, element e = (line e / ni, node 3 + e % ni),
-        // consecutive threads on consecutive nodes; the state words of kSplBatch
-        // elements are loaded before any is combined
-        const int ni = m - 6;
+        // consecutive threads on consecutive nodes
         const int ne = (int)min((unsigned)kSplL, nl - t0) * ni;
-        constexpr int kSplBatch = 4;
+        auto elem = [&](int e) { return ax.base(g, t0 + (unsigned)(e / ni)) + (unsigned)(3 + e % ni); };
+        // beside the solve: the state sums and mu(T), the state words of
+        // kSplBatch elements loaded before any is combined
+        auto state_sums = [&]() {
+            constexpr int nt = kSplThreads - 64;
+            for (int e0 = threadIdx.x - 64; e0 < ne; e0 += kSplBatch * nt) {
+                Raw5<St> raw[kSplBatch];
+                unsigned cc[kSplBatch];
+#pragma unroll
+                for (int k = 0; k < kSplBatch; k++) {
+                    cc[k] = elem(min(e0 + k * nt, ne - 1));  // past the end: repeat the last, unused
+                    raw[k].load(st, cc[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < kSplBatch; k++) {
+                    if (e0 + k * nt >= ne) continue;
+                    double rho, q2, P, T, iv, mu;
+                    cell_fast(raw[k], p, rho, q2, P, T, iv, mu);
+                    put(mu_pl, cc[k], mu);
+                    acc[0] += rho * q2;
+                    acc[12] += P; acc[13] += T; acc[14] += iv;
+                    const double dp = P - K[0], dT = T - K[1], dv = iv - K[2];
+                    acc[15] += dp; acc[16] += dT; acc[17] += dv;
+                    acc[18] += dp * dp; acc[19] += dT * dT; acc[20] += dv * dv;
+                }
+            }
+        };
+        // its closing barrier also orders the mu stores before the loads below
+        spl_group_solve(g, st, ax, t0, nl, inv, r3h, spl_tile, state_sums);
         for (int e0 = threadIdx.x; e0 < ne; e0 += kSplBatch * kSplThreads) {
-            Raw5<St> raw[kSplBatch];
+            double mu[kSplBatch];
             unsigned cc[kSplBatch];
 #pragma unroll
             for (int k = 0; k < kSplBatch; k++) {
-                const int e = min(e0 + k * kSplThreads, ne - 1);  // past the end: repeat the last, unused
-                const int l = e / ni, n = 3 + e % ni;
-                cc[k] = ax.base(g, t0 + (unsigned)l) + (unsigned)n;
-                raw[k].load(st, cc[k]);
+                cc[k] = elem(min(e0 + k * kSplThreads, ne - 1));
+                mu[k] = ld(mu_pl, cc[k]);
             }
 #pragma unroll
             for (int k = 0; k < kSplBatch; k++) {
                 const int e = e0 + k * kSplThreads;
-                if (e < ne) body_node(raw[k], cc[k], e / ni, 3 + e % ni, m);
+                if (e >= ne) continue;
+                const int l = e / ni, n = 3 + e % ni;
+                const double dz[3] = {spl_tile[(0 * m + n) * kSplL + l], spl_tile[(1 * m + n) * kSplL + l],
+                                      spl_tile[(2 * m + n) * kSplL + l]};
+                put(grad + (size_t)kPlDudz * g.ncell, cc[k], dz[0]);
+                put(grad + (size_t)kPlDvdz * g.ncell, cc[k], dz[1]);
+                sums.add(dz[0], mu[k], false, 0.0);  // dudz: its cross term dudz dwdx is added by the x pass
+                sums.add(dz[1], mu[k], false, 0.0);  // dvdz: dwdy dvdz by the y pass
+                sums.add(dz[2], mu[k], true, 0.0);
             }
         }
     }
@@ -892,14 +937,41 @@ __global__ void __launch_bounds__(kRedThreads) k_pstats(Geo g, St st, const doub
     block_reduce_store<kPSum, 0u>(acc, part);
 }
 
-// the PrintStats row: the kPStat sums, then the kStatB slots of pass B (rho:
-// 0, not reported; p, T, 1/rho: the sums of squared deviations about the
-// means, S2 - S1 (S1 / N) from the shifted sums)
-__global__ void __launch_bounds__(kRedThreads) k_pstats_final(int nb, const double* part, double N, double* out) {
+// The PrintStats row from nb partial rows: the kPStat sums, then the kStatB
+// slots of pass B (rho: 0, not reported; p, T, 1/rho: the sums of squared
+// deviations about the means, S2 - S1 (S1 / N) from the shifted sums).  The
+// rows are read as one flat array by kFinalThreads threads (a multiple of
+// kPSum): thread t sums elements t, t + kFinalThreads, ..., all of slot
+// t % kPSum, so consecutive threads read consecutive words and kFinalUnroll
+// loads of each thread are in flight together; the kFinalThreads / kPSum
+// values of a slot are then added in thread order.
+constexpr int kFinalRowsPerPass = 48, kFinalThreads = kPSum * kFinalRowsPerPass, kFinalUnroll = 4;
+__global__ void __launch_bounds__(1024) k_pstats_final(int nb, const double* part, double N, double* out) {
+    static_assert(kFinalThreads <= 1024, "one block");
+    __shared__ double sl[kPSum][kFinalRowsPerPass];
     __shared__ double r[kPSum];
-    reduce_rows<kPSum, 0u>(nb, part, r);
-    __syncthreads();
     const int t = threadIdx.x;
+    if (t < kFinalThreads) {
+        const long n = (long)nb * kPSum;
+        double v = 0.0;
+        long e = t;
+        for (; e + (kFinalUnroll - 1) * (long)kFinalThreads < n; e += kFinalUnroll * (long)kFinalThreads) {
+            double x[kFinalUnroll];
+#pragma unroll
+            for (int u = 0; u < kFinalUnroll; u++) x[u] = part[e + u * (long)kFinalThreads];
+#pragma unroll
+            for (int u = 0; u < kFinalUnroll; u++) v += x[u];
+        }
+        for (; e < n; e += kFinalThreads) v += part[e];
+        sl[t % kPSum][t / kPSum] = v;
+    }
+    __syncthreads();
+    if (t < kPSum) {
+        double v = sl[t][0];
+        for (int k = 1; k < kFinalRowsPerPass; k++) v += sl[t][k];
+        r[t] = v;
+    }
+    __syncthreads();
     if (t < kPStat) {
         out[t] = r[t];
     } else if (t < kPStat + kStatB) {
@@ -988,7 +1060,7 @@ hipError_t launch_print_stats_async(Ctx* h, int box, hipStream_t s, double* host
         }
         nb = launch_print_stats_st(h, st, p, s, &part);
     }
-    hipLaunchKernelGGL(k_pstats_final, dim3(1), dim3(kRedThreads), 0, s, nb, part,
+    hipLaunchKernelGGL(k_pstats_final, dim3(1), dim3(kFinalThreads), 0, s, nb, part,
                        (double)h->grid.nx * h->grid.ny * h->grid.nz, h->red_out);
     return hipMemcpyAsync(host_out, h->red_out, sizeof(double) * (kPStat + kStatB), hipMemcpyDeviceToHost, s);
 }
