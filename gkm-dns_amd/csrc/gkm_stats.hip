@@ -700,7 +700,7 @@ template <class St, int A, bool SUMS>
 __global__ void __launch_bounds__(kSplStreamThreads) k_spl_stream(Geo g, St st, const double* __restrict__ inv,
                                                    const double* __restrict__ wt, double r3h,
                                                    double* __restrict__ grad, double* __restrict__ part) {
-    static_assert(A != 2, "z lines go through k_spl_lines");
+    static_assert(A != 2, "z lines go through k_spl_lines_sums or k_spl_chunked");
     static_assert(kSplRing % kSplAhead == 0, "queue slots fixed within the unrolled ring");
     const SplAxis<A> ax(g);
     const unsigned nl = ax.n_in * ax.n_out;
