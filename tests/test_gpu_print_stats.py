@@ -86,6 +86,23 @@ def test_print_stats_long_lines(gk, orc, shape):
     _check(dev, ref, 1e-11)
 
 
+@pytest.mark.parametrize("shape", [(6, 5, 3), (5, 6, 4), (4, 5, 335), (5, 4, 334)])
+def test_print_stats_z_line_lengths(gk, orc, shape):
+    """The z lines are solved from both ends toward row m/2 (m = nz + 6 padded
+    nodes), the upper half one row longer when m is even: the shortest lines
+    the grid allows (m = 9, 10) and the longest solved whole in LDS (m = 341,
+    340), against the oracle."""
+    nx, ny, nz = shape
+    print("print_stats z line lengths", shape, flush=True)
+    grid, W = _rough_state(orc, nx, ny, nz, seed=nz + 11)
+    gas = _gas(gk)
+    with gk.Solver(nx, ny, nz, gas=gas) as s:
+        s.set_state(W)
+        dev = s.print_stats()
+    ref = orc.print_stats(grid, _orc_params(orc, gas), W)
+    _check(dev, ref, 1e-11)
+
+
 def test_print_stats_mixed(gk, orc):
     nx, ny, nz = 16, 12, 20
     print("print_stats mixed", (nx, ny, nz), flush=True)
