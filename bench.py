@@ -55,7 +55,8 @@ FLOPS_PROFILE = ROOT / "profiles" / "k_flux_256_valu_flops.md"
 
 
 # the kernel sources whose counters the committed summaries record
-KERNEL_SOURCES = ("gkm_device.h", "gkm_step.hip", "gkm_kernels.h", "gkm_erfcx.h")
+KERNEL_SOURCES = ("gkm_device.h", "gkm_step_dev.h", "gkm_step.hip", "gkm_step_ilp.hip", "gkm_step_mc.hip",
+                  "gkm_kernels.h", "gkm_erfcx.h", "Makefile")
 
 
 def kernel_source_sha1() -> str:
