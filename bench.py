@@ -466,10 +466,13 @@ def main():
             "note": "algorithmic = the reference scheme's operations (oracle op count, WENO once per interface); "
                     "the kernels reach it with fewer issued operations (co-moving-frame moments), so the fraction "
                     "is an effective rate.  The FP32 peak counts packed FP32, which this build does not use."}
-    valu["mixed"]["frac"] = valu["mixed"]["achieved"] / FP32_VEC_TFLOPS
+    # an effective rate (the algorithm's flops over the measured time), which can exceed the
+    # peak because the kernels issue fewer operations than the algorithm counts; the issued
+    # fraction below is the one measured against the hardware
+    valu["mixed"]["effective_frac"] = valu["mixed"]["achieved"] / FP32_VEC_TFLOPS
     if res_fp64:
         a64 = res_fp64["value"] * ALG_FLOPS_PER_CELL / 1e12
-        valu["fp64"] = {"achieved": a64, "peak": FP64_VEC_TFLOPS, "frac": a64 / FP64_VEC_TFLOPS}
+        valu["fp64"] = {"achieved": a64, "peak": FP64_VEC_TFLOPS, "effective_frac": a64 / FP64_VEC_TFLOPS}
     if flops32:
         tf = flops32 / (flux_ms * 1e-3) / 1e12
         issued = {"kernel": "k_flux (X, Y)", "achieved": tf, "peak": FP32_VEC_TFLOPS, "frac": tf / FP32_VEC_TFLOPS,
